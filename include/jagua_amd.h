@@ -1,0 +1,264 @@
+/* jagua_amd.h — C ABI of the MI355X (gfx950) collision-query engine.
+ *
+ * Drop-in boundary for the collision-query hot path of jagua-rs: the CDEngine / Hazard /
+ * HazardFilter / HazardCollector / SPSurrogate API, re-exposed as plain C so a Rust crate
+ * (or any FFI) can bind it with `extern "C"`.  Every entry point cites the reference
+ * interface it replaces (paths relative to jagua-rs/src/ and lbf/ in JeroenGar/jagua-rs).
+ *
+ * Conventions
+ *  - Return value: jg_status, 0 = JG_OK, negative = error (no entry point aborts).
+ *  - Geometry is IEEE binary32, the reference's `f32`; vertex arrays are xy-interleaved.
+ *  - Handles are opaque and not thread-safe; queries read the engine, updates write it.
+ *  - Query entry points take DEVICE pointers (resident in HBM) and a HIP stream passed as
+ *    `void*` (a hipStream_t; NULL = the default stream).  Outputs are valid after the
+ *    stream is synchronised.  jg_*_host variants take host pointers and synchronise.
+ *  - A 2x3 transform is the reference's Transformation matrix rows
+ *    (m00 m01 m02 m10 m11 m12), transformation.rs:131-139; points map as
+ *    x' = m00*x + m01*y + m02*1, y' = m10*x + m11*y + m12*1 (point.rs:26-31).
+ */
+#ifndef JAGUA_AMD_H
+#define JAGUA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t jg_status;
+#define JG_OK 0
+#define JG_ERR_INVALID_ARG (-1)
+#define JG_ERR_POLYGON (-2)          /* SPolygon::new bail! (simple_polygon.rs:42-51) */
+#define JG_ERR_NO_EXTERIOR (-3)      /* CDEngine::new expect (cd_engine.rs:44-48) */
+#define JG_ERR_NOT_REGISTERED (-4)   /* deregister expect (cd_engine.rs:78,91) */
+#define JG_ERR_SURROGATE (-5)        /* surrogate missing / ff_poles slice (sp_surrogate.rs:56) */
+#define JG_ERR_HIP (-6)              /* a HIP runtime call failed */
+#define JG_ERR_DEPTH (-7)            /* quadtree depth above JG_MAX_QT_DEPTH */
+#define JG_ERR_CAPACITY (-8)         /* an output buffer is too small */
+#define JG_ERR_INTERNAL (-9)
+
+/* Per-query status written by the detect entry points. */
+#define JG_Q_CLEAR 0                 /* no collision */
+#define JG_Q_COLLISION 1             /* collision */
+#define JG_Q_PANIC_FILTERED_ENTIRE 2 /* the reference reaches unreachable!() here:
+                                        a filtered Entire hazard in the virtual root
+                                        (cd_engine.rs:176-181); reported, not aborted */
+
+#define JG_MAX_QT_DEPTH 12
+
+/* Query composition flags (detect). Default (0) = SURROGATE|POLY. */
+#define JG_QUERY_SURROGATE 1u        /* CDEngine::detect_surrogate_collision (cd_engine.rs:204-223) */
+#define JG_QUERY_POLY 2u             /* CDEngine::detect_poly_collision (cd_engine.rs:160-197) */
+
+typedef struct jg_spolygon jg_spolygon;   /* SPolygon (+ SPSurrogate), simple_polygon.rs:24-37 */
+typedef struct jg_cde jg_cde;             /* CDEngine, cd_engine.rs:21-31 */
+typedef struct jg_snapshot jg_snapshot;   /* CDESnapshot, cd_engine.rs:356-359 */
+typedef struct jg_plan jg_plan;           /* batched persistent query plan (no reference counterpart) */
+
+typedef struct {                          /* SPSurrogateConfig, sp_surrogate.rs:120-131 */
+    uint32_t n_pole_limits_n[3];
+    float n_pole_limits_cov[3];
+    uint32_t n_ff_poles;
+    uint32_t n_ff_piers;
+} jg_surrogate_config;
+
+typedef struct {                          /* CDEConfig, cd_engine.rs:345-353 */
+    uint32_t quadtree_depth;
+    uint32_t cd_threshold;
+    jg_surrogate_config item_surrogate_config;
+} jg_cde_config;
+
+typedef enum {                            /* HazardEntity, hazards/hazard.rs:38-51 */
+    JG_ENTITY_PLACED_ITEM = 0,            /* a = item id, b = placed-item key, dt = DTransformation */
+    JG_ENTITY_EXTERIOR = 1,
+    JG_ENTITY_HOLE = 2,                   /* a = idx */
+    JG_ENTITY_INFERIOR_QUALITY_ZONE = 3   /* a = quality, b = idx */
+} jg_entity_kind;
+
+typedef struct {
+    uint32_t kind;
+    uint32_t _pad;
+    uint64_t a;
+    uint64_t b;
+    float dt[3];                          /* rotation, tx, ty */
+    uint32_t _pad2;
+} jg_hazard_entity;
+
+typedef struct {                          /* HazKey (slotmap key), hazard.rs:8-11 */
+    uint32_t idx;
+    uint32_t version;
+} jg_hazkey;
+
+/* HazardFilter (hazards/filter.rs:7-9) */
+#define JG_FILTER_NONE 0u                 /* NoFilter (filter.rs:57-65) */
+#define JG_FILTER_KEYS 1u                 /* HazKeyFilter / HazKey self-filter (filter.rs:13-55) */
+#define JG_FILTER_HASHED 2u               /* per-query pseudo-random subset: hazard slot s is
+                                             irrelevant for query q iff
+                                             jg_filter_hash(seed, q, s) < threshold */
+typedef struct {
+    uint32_t kind;
+    uint32_t n_keys;
+    const jg_hazkey* keys;                /* host pointer, JG_FILTER_KEYS */
+    uint32_t seed;
+    uint32_t threshold;
+} jg_hazard_filter;
+
+/* The hashed filter predicate and the order-independent set hash, specified bit-exactly so
+ * any host can reproduce them:
+ *   fmix32(h)      = murmur3 finaliser (h^=h>>16; h*=0x85ebca6b; h^=h>>13; h*=0xc2b2ae35; h^=h>>16)
+ *   jg_filter_hash = fmix32(seed ^ fmix32(lo32(q) ^ fmix32(hi32(q) + 0x9e3779b9)) ^ (s * 0x85ebca77))
+ *   mix64(z)       = splitmix64 finaliser
+ *   set hash       = sum over slots s in the set of mix64(s)   (mod 2^64)
+ */
+
+/* ---------------------------------------------------------------- SPolygon / SPSurrogate */
+
+/* SPolygon::new (simple_polygon.rs:41-71): CCW normalisation, area, diameter, bbox, POI. */
+jg_status jg_spolygon_new(const float* xy, uint32_t n_vertices, jg_spolygon** out);
+/* SPolygon::generate_surrogate (simple_polygon.rs:73-80) -> SPSurrogate::new (sp_surrogate.rs:33-53). */
+jg_status jg_spolygon_generate_surrogate(jg_spolygon* poly, const jg_surrogate_config* cfg);
+/* Transformable::transform_clone with DTransformation::compose (placed_item.rs:27-36,
+ * simple_polygon.rs:195-224): vertices, POI and surrogate transformed, bbox regenerated. */
+jg_status jg_spolygon_transform_clone(const jg_spolygon* poly, const float dt[3], jg_spolygon** out);
+void jg_spolygon_free(jg_spolygon* poly);
+
+typedef struct {
+    uint32_t n_vertices;
+    uint32_t n_poles;                     /* all surrogate poles (0 without surrogate) */
+    uint32_t n_piers;
+    uint32_t n_ff_poles;                  /* SPSurrogate::ff_poles() length */
+    float area, diameter;
+    float bbox[4];                        /* x_min y_min x_max y_max */
+    float poi[3];                         /* cx cy r */
+    float convex_hull_area;
+} jg_spolygon_info;
+jg_status jg_spolygon_get_info(const jg_spolygon* poly, jg_spolygon_info* out);
+jg_status jg_spolygon_get_vertices(const jg_spolygon* poly, float* xy_out);       /* [n][2] */
+jg_status jg_spolygon_get_poles(const jg_spolygon* poly, float* out);             /* [n_poles][3] */
+jg_status jg_spolygon_get_piers(const jg_spolygon* poly, float* out);             /* [n_piers][4] */
+
+/* DTransformation::compose (d_transformation.rs:45-47 -> transformation.rs:91-101,131-139):
+ * dt[i] = (rotation, tx, ty) -> mat6[i]; sin/cos come from the host libm (as in Rust std). */
+jg_status jg_transforms_from_dt(const float* dt, uint64_t n, float* mat6_out);
+
+/* ---------------------------------------------------------------- CDEngine */
+
+/* CDEngine::new (cd_engine.rs:34-56).  shapes are copied; dynamic[i] is Hazard::dynamic. */
+jg_status jg_cde_new(const float bbox[4], const jg_cde_config* cfg, const jg_hazard_entity* entities,
+                     const jg_spolygon* const* shapes, const uint8_t* dynamic, uint32_t n, jg_cde** out);
+void jg_cde_free(jg_cde* cde);
+/* CDEngine::register_hazard (cd_engine.rs:59-69); the new key is written to *key_out (may be NULL). */
+jg_status jg_cde_register_hazard(jg_cde* cde, const jg_hazard_entity* entity, const jg_spolygon* shape,
+                                 uint8_t dynamic, jg_hazkey* key_out);
+/* CDEngine::deregister_hazard_by_entity / _by_key (cd_engine.rs:72-96). */
+jg_status jg_cde_deregister_hazard_by_entity(jg_cde* cde, const jg_hazard_entity* entity);
+jg_status jg_cde_deregister_hazard_by_key(jg_cde* cde, jg_hazkey key);
+/* CDEngine::save / restore (cd_engine.rs:98-149). */
+jg_status jg_cde_save(const jg_cde* cde, jg_snapshot** out);
+jg_status jg_cde_restore(jg_cde* cde, const jg_snapshot* snap);
+void jg_snapshot_free(jg_snapshot* snap);
+/* CDEngine::bbox (cd_engine.rs:329-331) and the exterior key (cd_engine.rs:29-30). */
+jg_status jg_cde_bbox(const jg_cde* cde, float out[4]);
+jg_status jg_cde_hkey_exterior(const jg_cde* cde, jg_hazkey* out);
+
+typedef struct {
+    uint64_t n_nodes, n_entries, n_edge_refs, n_hazard_vertices, n_slots;
+    uint64_t device_bytes;
+    double flatten_ms, upload_ms;
+} jg_cde_stats;
+/* Flattens the quadtree to its BFS structure-of-arrays form and uploads it to HBM when it
+ * changed since the last upload (queries do this implicitly). */
+jg_status jg_cde_sync(jg_cde* cde);
+jg_status jg_cde_get_stats(const jg_cde* cde, jg_cde_stats* out);
+/* The flattened tree in BFS order (same record layout as the oracle's tree dump):
+ * node records [8 x u32]: level, n_children (0|4), n_hazards, n_active_edges, bbox (4 x f32 bits);
+ * hazard records [8 x u32]: slot idx, presence (1 Partial, 2 Entire), n_edges, 0, ff_bbox.
+ * Call with NULL buffers to obtain *n_nodes and *n_haz. */
+jg_status jg_cde_dump(const jg_cde* cde, uint32_t* nodes, uint64_t nodes_cap, uint32_t* haz, uint64_t haz_cap,
+                      uint64_t* n_nodes, uint64_t* n_haz);
+
+/* ---------------------------------------------------------------- collision queries */
+
+/* One collision query per transform i, as lbf samples them (lbf/src/opt/search.rs:41-57) and
+ * ci_bench composes them (lbf/benches/ci_bench.rs:107-120):
+ *   status[i] = detect_surrogate_collision(item.surrogate, T_i, filter)  (cd_engine.rs:204-223)
+ *            || detect_poly_collision(item transformed by T_i, filter)   (cd_engine.rs:160-197)
+ * item must carry a surrogate when JG_QUERY_SURROGATE is set.
+ * d_mat6: device [n][6] f32; d_status: device [n] u8 (JG_Q_*). */
+jg_status jg_detect_batch(jg_cde* cde, const jg_spolygon* item, const float* d_mat6, uint64_t n,
+                          const jg_hazard_filter* filter, uint32_t flags, uint8_t* d_status, void* stream);
+
+/* One hazard-collection query per transform i (ci_bench.rs:56-69):
+ *   set_i = collect_surrogate_collisions (cd_engine.rs:296-310)
+ *         ∪ collect_poly_collisions       (cd_engine.rs:261-289)
+ * BasicHazardCollector (collector.rs:44-67) is returned as, per query: d_count[i] (set size),
+ * d_hash[i] (set hash, see above) and the first `capacity` slot indices in d_keys[i*capacity..]
+ * in ascending order (d_keys may be NULL).  Sets larger than `capacity` are still counted
+ * and hashed exactly. */
+jg_status jg_collect_batch(jg_cde* cde, const jg_spolygon* item, const float* d_mat6, uint64_t n,
+                           uint32_t capacity, uint32_t* d_count, uint64_t* d_hash, uint32_t* d_keys,
+                           void* stream);
+
+/* Host-pointer conveniences of the two calls above (copy in, run, copy out, synchronise). */
+jg_status jg_detect_batch_host(jg_cde* cde, const jg_spolygon* item, const float* mat6, uint64_t n,
+                               const jg_hazard_filter* filter, uint32_t flags, uint8_t* status);
+jg_status jg_collect_batch_host(jg_cde* cde, const jg_spolygon* item, const float* mat6, uint64_t n,
+                                uint32_t capacity, uint32_t* count, uint64_t* hash, uint32_t* keys);
+
+/* ---------------------------------------------------------------- batched persistent form */
+
+/* A set of queries: one engine, one item, a contiguous range of transforms in d_mat6 and
+ * of outputs.  Many sets (many layouts, many items) run in ONE persistent launch fed by a
+ * device work queue.  No reference counterpart: it is lbf's sampled placement evaluation
+ * (search.rs:41-101) batched across items and layouts. */
+typedef struct {
+    jg_cde* cde;
+    const jg_spolygon* item;
+    uint64_t first;                       /* index of the set's first transform / output */
+    uint64_t count;
+    jg_hazard_filter filter;
+} jg_query_set;
+
+typedef struct {
+    uint32_t chunk;                       /* transforms per work unit (0 = autotuned) */
+    uint32_t workgroups;                  /* persistent workgroups (0 = autotuned) */
+    uint32_t lanes_per_query;             /* 0 = per item, from its vertex count */
+    uint32_t flags;                       /* JG_QUERY_* */
+} jg_plan_config;
+
+jg_status jg_plan_create(const jg_query_set* sets, uint32_t n_sets, const jg_plan_config* cfg, jg_plan** out);
+void jg_plan_free(jg_plan* plan);
+/* d_mat6 [total][6] and d_status [total] are device arrays indexed by jg_query_set::first + i. */
+jg_status jg_plan_detect(jg_plan* plan, const float* d_mat6, uint8_t* d_status, void* stream);
+/* The same launch captured once in a hipGraph and replayed (launch overhead amortised). */
+jg_status jg_plan_detect_graph(jg_plan* plan, const float* d_mat6, uint8_t* d_status, uint32_t replays,
+                               void* stream);
+
+/* ---------------------------------------------------------------- overlap gradient */
+
+/* Pole-pair overlap quantity (north-star extension; no reference counterpart; its primitives
+ * are Circle::separation_distance, circle.rs:184-200):
+ *   Q = sum over candidate item pairs (a,b), poles i of a, j of b, of max(0, r_i + r_j - |c_i - c_j|)^2
+ *     + sum over poles i, container edges e, of max(0, r_i - dist(c_i, e))^2
+ * with c_i = R(theta_a) p_i + (x_a, y_a).  Forward writes Q (device f32[1]); backward writes
+ * dQ/d(x, y, theta) per item (device f32[n_items][3]).  Pole transforms are recomputed in the
+ * backward pass, not stored.
+ * d_pose [n_items][3] (x, y, theta); d_poles [total][3] local (px, py, r);
+ * d_pole_off [n_items+1]; d_pairs [n_pairs][2] item indices (a < b);
+ * d_cedges [n_cedges][4] container edges (x1 y1 x2 y2). */
+jg_status jg_overlap_fwd(const float* d_pose, const float* d_poles, const uint32_t* d_pole_off, uint32_t n_items,
+                         const uint32_t* d_pairs, uint64_t n_pairs, const float* d_cedges, uint32_t n_cedges,
+                         float* d_q, void* stream);
+jg_status jg_overlap_bwd(const float* d_pose, const float* d_poles, const uint32_t* d_pole_off, uint32_t n_items,
+                         const uint32_t* d_pairs, uint64_t n_pairs, const float* d_cedges, uint32_t n_cedges,
+                         float* d_grad, void* stream);
+
+/* Version / build info. */
+const char* jg_version(void);
+/* Message of the last error on this thread ("" when none). */
+const char* jg_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* JAGUA_AMD_H */

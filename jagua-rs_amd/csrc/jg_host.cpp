@@ -1,0 +1,608 @@
+// Host build of the collision engine: SPolygon construction with its surrogate, the hazard
+// slot map, the quadtree node pool under register/deregister, and the BFS flattener.
+// Arithmetic follows the reference's f32 operation order (compiled -ffp-contract=off, no
+// fast-math, sinf/cosf as run-time libm calls).
+#include "jg_host.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+namespace jg {
+
+// ---------------------------------------------------------------------------------------
+// polygons
+// ---------------------------------------------------------------------------------------
+
+// transformation.rs:131-139 (sin_cos = libm sinf, cosf)
+Mat mat_from_dt(float rotation, float tx, float ty) {
+    float s = sinf(rotation);
+    float c = cosf(rotation);
+    return Mat{c, -s, tx, s, c, ty};
+}
+
+static float signed_area(const std::vector<float2>& v) {           // simple_polygon.rs:137-150
+    float sigma = 0.0f;
+    const size_t n = v.size();
+    for (size_t i = 0; i < n; i++) {
+        const float2 a = v[i], b = v[(i + 1) % n];
+        sigma += (a.y + b.y) * (a.x - b.x);
+    }
+    return 0.5f * sigma;
+}
+
+static bool bbox_of(const std::vector<float2>& v, float4& out) {   // simple_polygon.rs:122-133
+    float4 b = make_float4(FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX);
+    for (const float2& p : v) {
+        b.x = mn(b.x, p.x);
+        b.y = mn(b.y, p.y);
+        b.z = mx(b.z, p.x);
+        b.w = mx(b.w, p.y);
+    }
+    out = b;
+    return b.x < b.z && b.y < b.w;
+}
+
+bool polygon_has_point(const Polygon& p, float x, float y) {        // simple_polygon.rs:255-292
+    if (!box_has_point(p.bbox, x, y)) return false;
+    const float ray_x = p.bbox.z + (p.bbox.z - p.bbox.x);
+    int cnt = 0;
+    const uint32_t n = p.n();
+    for (uint32_t i = 0; i < n; i++) {
+        const float2 s = p.v[i], e = p.v[i + 1 == n ? 0 : i + 1];
+        cnt += pip_edge(x, y, ray_x, s.x, s.y, e.x, e.y);
+    }
+    return (cnt & 1) == 1;
+}
+
+// Monotone-chain hull (convex_hull.rs:31-64): stable x-sort, lower then upper chain.
+static std::vector<float2> hull_points(std::vector<float2> pts) {
+    std::stable_sort(pts.begin(), pts.end(), [](const float2& a, const float2& b) { return a.x < b.x; });
+    auto turn = [](float2 a, float2 b, float2 c) { return (b.x - a.x) * (c.y - a.y) - (b.y - a.y) * (c.x - a.x); };
+    auto chain = [&](auto begin, auto end) {
+        std::vector<float2> h;
+        for (auto it = begin; it != end; ++it) {
+            while (h.size() >= 2 && turn(h[h.size() - 2], h.back(), *it) <= 0.0f) h.pop_back();
+            h.push_back(*it);
+        }
+        h.pop_back();
+        return h;
+    };
+    std::vector<float2> lo = chain(pts.begin(), pts.end());
+    std::vector<float2> up = chain(pts.rbegin(), pts.rend());
+    lo.insert(lo.end(), up.begin(), up.end());
+    return lo;
+}
+
+static float sq_dist(float2 a, float2 b) {                          // point.rs:49-52
+    const float dx = a.x - b.x, dy = a.y - b.y;
+    return dx * dx + dy * dy;
+}
+
+// Edge::distance_to(Point) (edge.rs:146-159)
+static float edge_point_dist(float4 e, float px, float py) {
+    const float a = px - e.x, b = py - e.y, c = e.z - e.x, d = e.w - e.y;
+    const float dot = a * c + b * d;
+    const float len_sq = c * c + d * d;
+    float param = -1.0f;
+    if (len_sq != 0.0f) param = dot / len_sq;
+    float xx, yy;
+    if (param < 0.0f) { xx = e.x; yy = e.y; }
+    else if (param > 1.0f) { xx = e.z; yy = e.w; }
+    else { xx = e.x + param * c; yy = e.y + param * d; }
+    const float dx = px - xx, dy = py - yy;
+    return sqrtf(dx * dx + dy * dy);
+}
+
+// Pole of inaccessibility search (fail_fast/pole.rs:52-135): breadth-first over a quadtree of
+// the square bbox, 10 levels, pruned by radius + distance.
+static bool compute_pole(const Polygon& shape, const std::vector<float4>& poles, float4& out) {
+    struct Cell { float4 box; int level; float radius, dist; };
+    auto make_cell = [&](float4 box, int level) {
+        const float dx = box.z - box.x, dy = box.w - box.y;
+        const float radius = sqrtf(dx * dx + dy * dy) / 2.0f;
+        const float cx = (box.x + box.z) / 2.0f, cy = (box.y + box.w) / 2.0f;
+        bool inside = polygon_has_point(shape, cx, cy);
+        if (inside)
+            for (const float4& c : poles)
+                if (sq_dist(make_float2(cx, cy), make_float2(c.x, c.y)) <= c.z * c.z) { inside = false; break; }
+        float d = FLT_MAX;
+        for (uint32_t i = 0; i < shape.n(); i++) d = mn(d, edge_point_dist(shape.edge(i), cx, cy));
+        for (const float4& c : poles) {
+            const float ddx = cx - c.x, ddy = cy - c.y;
+            const float dc = sqrtf(ddx * ddx + ddy * ddy);
+            d = mn(d, dc <= c.z ? c.z - dc : dc - c.z);             // circle.rs:154-163
+        }
+        return Cell{box, level, radius, inside ? d : -d};
+    };
+    std::vector<Cell> fifo;
+    fifo.push_back(make_cell(box_inflate_to_square(shape.bbox), 10));
+    bool have = false;
+    float4 best{};
+    for (size_t head = 0; head < fifo.size(); head++) {
+        const Cell cell = fifo[head];
+        float best_r = have ? best.z : 0.0f;
+        if (cell.dist > best_r) {
+            best = make_float4((cell.box.x + cell.box.z) / 2.0f, (cell.box.y + cell.box.w) / 2.0f, cell.dist, 0.0f);
+            have = true;
+            best_r = cell.dist;
+        }
+        if (cell.radius + cell.dist > best_r && cell.level > 0)
+            for (int k = 0; k < 4; k++) fifo.push_back(make_cell(box_quadrant(cell.box, k), cell.level - 1));
+        if (head > 4096 && head * 2 > fifo.size()) {             // keep the FIFO compact
+            fifo.erase(fifo.begin(), fifo.begin() + (long)head + 1);
+            head = (size_t)-1;
+        }
+    }
+    out = best;
+    return have;
+}
+
+bool polygon_new(const float* xy, uint32_t n, Polygon& out, std::string& err) {   // simple_polygon.rs:41-71
+    if (n < 3) { err = "simple polygon must have at least 3 points"; return false; }
+    std::vector<float2> v(n);
+    for (uint32_t i = 0; i < n; i++) v[i] = make_float2(xy[2 * i], xy[2 * i + 1]);
+    {
+        std::vector<std::pair<float, float>> s(n);
+        for (uint32_t i = 0; i < n; i++) s[i] = {v[i].x, v[i].y};
+        std::sort(s.begin(), s.end());
+        for (uint32_t i = 1; i < n; i++)
+            if (s[i] == s[i - 1]) { err = "simple polygon should not contain duplicate points"; return false; }
+    }
+    float area = signed_area(v);
+    if (area == 0.0f) { err = "simple polygon has no area"; return false; }
+    if (area < 0.0f) {
+        std::reverse(v.begin(), v.end());
+        area = -area;
+    }
+    Polygon p;
+    // diameter: largest squared distance over convex-hull pairs (:107-120)
+    std::vector<float2> h = hull_points(v);
+    if (h.size() < 2) { err = "convex hull is empty"; return false; }
+    float best = -1.0f;
+    for (size_t i = 0; i < h.size(); i++)
+        for (size_t j = i + 1; j < h.size(); j++) best = mx(best, sq_dist(h[i], h[j]));
+    p.diameter = sqrtf(best);
+    if (!bbox_of(v, p.bbox)) { err = "invalid rectangle"; return false; }
+    p.v = std::move(v);
+    p.area = signed_area(p.v);     // calculate_poi's helper polygon recomputes the area
+    float4 poi;
+    if (!compute_pole(p, {}, poi)) { err = "no pole found"; return false; }
+    p.poi = poi;
+    p.area = area;
+    out = std::move(p);
+    return true;
+}
+
+// generate_surrogate_poles (pole.rs:12-47)
+static bool surrogate_poles(const Polygon& shape, const jg_surrogate_config& cfg, std::vector<float4>& out) {
+    const float PI_F = 3.14159265358979323846f;
+    out.assign(1, shape.poi);
+    float covered = shape.poi.z * shape.poi.z * PI_F;
+    for (;;) {
+        float4 next;
+        if (!compute_pole(shape, out, next)) return false;
+        covered += next.z * next.z * PI_F;
+        out.push_back(next);
+        const float coverage = covered / shape.area;
+        bool any = false;
+        uint32_t limit = 0;
+        for (int i = 0; i < 3; i++)
+            if (coverage > cfg.n_pole_limits_cov[i] && (!any || cfg.n_pole_limits_n[i] < limit)) {
+                limit = cfg.n_pole_limits_n[i];
+                any = true;
+            }
+        if (any && out.size() >= limit) return true;
+        if (out.size() >= 1000) return false;
+    }
+}
+
+// ndarray 0.17 linspace: a + step * i with step = (b - a) / (n - 1)
+static float lin(float a, float b, size_t n, size_t i) {
+    const float step = n > 1 ? (b - a) / (float)(n - 1) : 0.0f;
+    return a + step * (float)i;
+}
+
+// generate_piers (fail_fast/piers.rs:25-226), release-build constants (:16-20)
+static bool surrogate_piers(const Polygon& shape, uint32_t n, const std::vector<float4>& poles,
+                            std::vector<float4>& out) {
+    out.clear();
+    if (n == 0) return true;
+    const size_t RAYS = 200, ANGLES = 90, GRID = 100;
+    const float TRIM = 0.999f, INFL = 0.10f, PI_F = 3.14159265358979323846f;
+    const float4 bb = shape.bbox;
+    const float4 sq = box_inflate_to_square(bb);
+    // SPolygon::centroid (simple_polygon.rs:173-192)
+    float gx = 0.0f, gy = 0.0f;
+    for (uint32_t i = 0; i < shape.n(); i++) {
+        const float2 a = shape.v[i], b = shape.v[i + 1 == shape.n() ? 0 : i + 1];
+        gx += (a.x + b.x) * (a.x * b.y - b.x * a.y);
+        gy += (a.y + b.y) * (a.x * b.y - b.x * a.y);
+    }
+    gx /= 6.0f * shape.area;
+    gy /= 6.0f * shape.area;
+    const float sq_h = sq.w - sq.y;
+    const float2 r0 = make_float2(gx, gy - 2.0f * sq_h), r1 = make_float2(gx, gy + 2.0f * sq_h);
+    const float dx = (sq.z - sq.x) / (float)RAYS;
+    std::vector<float4> clipped;
+    for (size_t ai = 0; ai < ANGLES; ai++) {
+        const float ang = lin(0.0f, PI_F, ANGLES + 1, ai);
+        const float s = sinf(ang), c = cosf(ang);
+        for (size_t i = 0; i < RAYS; i++) {
+            const float tx = sq.x + dx * (float)i;
+            // rot_m(angle) . transl_m(tx, 0) (transformation.rs:156-178), term by term
+            const float m00 = c * 1.0f + -s * 0.0f + 0.0f * 0.0f, m01 = c * 0.0f + -s * 1.0f + 0.0f * 0.0f;
+            const float m02 = c * tx + -s * 0.0f + 0.0f * 1.0f;
+            const float m10 = s * 1.0f + c * 0.0f + 0.0f * 0.0f, m11 = s * 0.0f + c * 1.0f + 0.0f * 0.0f;
+            const float m12 = s * tx + c * 0.0f + 0.0f * 1.0f;
+            const Mat m{m00, m01, m02, m10, m11, m12};
+            const float2 a = xform(m, r0.x, r0.y), b = xform(m, r1.x, r1.y);
+            if (polygon_has_point(shape, a.x, a.y) || polygon_has_point(shape, b.x, b.y)) return false;
+            // clip (:124-151): intersections sorted by distance from the ray start
+            std::vector<std::pair<float, float2>> hits;
+            for (uint32_t k = 0; k < shape.n(); k++) {
+                const float4 e = shape.edge(k);
+                if (!edge_edge(e, make_float4(a.x, a.y, b.x, b.y))) continue;
+                const float x1 = e.x, y1 = e.y, x2 = e.z, y2 = e.w, x3 = a.x, y3 = a.y, x4 = b.x, y4 = b.y;
+                const float t = ((x2 - x4) * (y4 - y3) - (y2 - y4) * (x4 - x3)) /
+                                ((x2 - x1) * (y4 - y3) - (y2 - y1) * (x4 - x3));
+                const float2 loc = make_float2(x2 + t * (x1 - x2), y2 + t * (y1 - y2));
+                hits.push_back({sqrtf(sq_dist(a, loc)), loc});
+            }
+            std::stable_sort(hits.begin(), hits.end(),
+                             [](const std::pair<float, float2>& u, const std::pair<float, float2>& w) { return u.first < w.first; });
+            for (size_t k = 0; k + 1 < hits.size(); k += 2) {
+                const float2 st = hits[k].second, en = hits[k + 1].second;
+                if (st.x == en.x && st.y == en.y) continue;
+                const float ddx = en.x - st.x, ddy = en.y - st.y;   // Edge::scale (edge.rs:39-46)
+                clipped.push_back(make_float4(st.x - ddx * (TRIM - 1.0f) / 2.0f, st.y - ddy * (TRIM - 1.0f) / 2.0f,
+                                              en.x + ddx * (TRIM - 1.0f) / 2.0f, en.y + ddy * (TRIM - 1.0f) / 2.0f));
+            }
+        }
+    }
+    std::vector<float2> grid;
+    for (size_t i = 0; i < GRID; i++)
+        for (size_t j = 0; j < GRID; j++) {
+            const float2 p = make_float2(lin(sq.x, sq.z, GRID, i), lin(sq.y, sq.w, GRID, j));
+            if (!polygon_has_point(shape, p.x, p.y)) continue;
+            bool free = true;
+            for (const float4& c : poles)
+                if (sq_dist(p, make_float2(c.x, c.y)) <= c.z * c.z) { free = false; break; }
+            if (free) grid.push_back(p);
+        }
+    const float infl = INFL * (sq.z - sq.x);
+    const float bw = bb.z - bb.x, bh = bb.w - bb.y;
+    const float forfeit = sqrtf((bw * bw) * (bh * bh));
+    for (uint32_t it = 0; it < n; it++) {
+        std::vector<float> d_ray(grid.size()), d_pole(grid.size());
+        for (size_t g = 0; g < grid.size(); g++) {
+            float a = forfeit, b = forfeit;
+            for (const float4& r : out) a = mn(a, edge_point_dist(r, grid[g].x, grid[g].y));
+            for (const float4& c : poles) {      // Circle::distance_to (circle.rs:136-147)
+                const float ddx = grid[g].x - c.x, ddy = grid[g].y - c.y;
+                const float sqd = ddx * ddx + ddy * ddy;
+                b = mn(b, sqd < c.z * c.z ? 0.0f : sqrtf(sqd) - c.z);
+            }
+            d_ray[g] = a;
+            d_pole[g] = b;
+        }
+        if (clipped.empty()) return false;
+        size_t best_i = 0;
+        float best_loss = 0.0f;
+        for (size_t r = 0; r < clipped.size(); r++) {
+            float loss = -0.0f;
+            for (size_t g = 0; g < grid.size(); g++) {
+                const float dn = mn(d_ray[g], edge_point_dist(clipped[r], grid[g].x, grid[g].y));
+                const float v = dn < infl ? mn(d_pole[g], dn) : d_pole[g];
+                loss += v * v;
+            }
+            if (r == 0 || loss < best_loss) { best_loss = loss; best_i = r; }
+        }
+        out.push_back(clipped[best_i]);
+    }
+    return true;
+}
+
+bool polygon_generate_surrogate(Polygon& p, const jg_surrogate_config& cfg, std::string& err) {  // sp_surrogate.rs:33-53
+    if (p.sur.present && std::memcmp(&p.sur.cfg, &cfg, sizeof cfg) == 0) return true;
+    Surrogate s;
+    std::vector<float2> h = hull_points(p.v);
+    for (const float2& q : h)
+        for (uint32_t i = 0; i < p.n(); i++)
+            if (p.v[i].x == q.x && p.v[i].y == q.y) { s.hull_idx.push_back(i); break; }
+    s.hull_area = signed_area(h);
+    if (!surrogate_poles(p, cfg, s.poles)) { err = "pole generation failed"; return false; }
+    const size_t n_ff = std::min<size_t>(cfg.n_ff_poles, s.poles.size());
+    std::vector<float4> rel(s.poles.begin(), s.poles.begin() + (long)n_ff);
+    if (!surrogate_piers(p, cfg.n_ff_piers, rel, s.piers)) { err = "pier generation failed"; return false; }
+    s.cfg = cfg;
+    s.present = true;
+    p.sur = std::move(s);
+    return true;
+}
+
+Polygon polygon_transform(const Polygon& p, const Mat& m) {        // simple_polygon.rs:195-224
+    Polygon r = p;
+    for (float2& q : r.v) q = xform(m, q.x, q.y);
+    const float2 c = xform(m, p.poi.x, p.poi.y);
+    r.poi.x = c.x;
+    r.poi.y = c.y;
+    if (r.sur.present) {
+        for (float4& c4 : r.sur.poles) {
+            const float2 t = xform(m, c4.x, c4.y);
+            c4.x = t.x;
+            c4.y = t.y;
+        }
+        for (float4& e : r.sur.piers) {
+            const float2 a = xform(m, e.x, e.y), b = xform(m, e.z, e.w);
+            e = make_float4(a.x, a.y, b.x, b.y);
+        }
+    }
+    bbox_of(r.v, r.bbox);
+    return r;
+}
+
+// ---------------------------------------------------------------------------------------
+// hazards
+// ---------------------------------------------------------------------------------------
+bool entity_eq(const jg_hazard_entity& a, const jg_hazard_entity& b) {
+    return a.kind == b.kind && a.a == b.a && a.b == b.b && std::memcmp(a.dt, b.dt, sizeof a.dt) == 0;
+}
+
+jg_hazkey SlotMap::insert(HazardRec r) {
+    uint32_t idx;
+    if (free_head < slots.size()) {
+        idx = free_head;
+        free_head = slots[idx].next_free;
+    } else {
+        idx = (uint32_t)slots.size();
+        slots.emplace_back();
+        free_head = idx + 1;
+    }
+    Slot& s = slots[idx];
+    s.version += 1;
+    s.occ = true;
+    s.rec = std::move(r);
+    return jg_hazkey{idx, s.version};
+}
+
+HazardRec SlotMap::remove(jg_hazkey k) {
+    Slot& s = slots[k.idx];
+    HazardRec r = std::move(s.rec);
+    s.rec = HazardRec{};
+    s.occ = false;
+    s.version += 1;
+    s.next_free = free_head;
+    free_head = k.idx;
+    return r;
+}
+
+// ---------------------------------------------------------------------------------------
+// quadtree (qt_node.rs, qt_hazard.rs, qt_partial_hazard.rs, qt_hazard_vec.rs)
+// ---------------------------------------------------------------------------------------
+void QuadTree::init(uint8_t depth, float4 bbox, uint8_t cdt) {
+    pool.assign(1, QNode{});
+    pool[0].level = depth;
+    pool[0].bbox = bbox;
+    free_blocks.clear();
+    cd_threshold = cdt;
+}
+
+int32_t QuadTree::alloc_children(int32_t parent) {
+    int32_t base;
+    if (!free_blocks.empty()) {
+        base = free_blocks.back();
+        free_blocks.pop_back();
+    } else {
+        base = (int32_t)pool.size();
+        pool.resize(pool.size() + 4);
+    }
+    const float4 pb = pool[parent].bbox;
+    const uint8_t lv = (uint8_t)(pool[parent].level - 1);
+    for (int k = 0; k < 4; k++) {
+        QNode& c = pool[base + k];
+        c = QNode{};
+        c.level = lv;
+        c.bbox = box_quadrant(pb, k);
+        if (!(c.bbox.x < c.bbox.z && c.bbox.y < c.bbox.w)) c.bbox = c.bbox;  // degenerate boxes are kept as is
+    }
+    pool[parent].child = base;
+    return base;
+}
+
+void QuadTree::free_subtree_children(int32_t node) {
+    const int32_t base = pool[node].child;
+    if (base < 0) return;
+    for (int k = 0; k < 4; k++) free_subtree_children(base + k);
+    for (int k = 0; k < 4; k++) pool[base + k] = QNode{};
+    free_blocks.push_back(base);
+    pool[node].child = -1;
+}
+
+static void add_entry(QNode& n, QEntry e) {                         // qt_hazard_vec.rs:24-43
+    n.n_active_edges += e.presence == PRES_PARTIAL ? e.edges.size() : 0;
+    if (e.presence == PRES_ENTIRE) {
+        size_t pos = 0;
+        while (pos < n.haz.size() && n.haz[pos].presence == PRES_ENTIRE) pos++;
+        n.haz.insert(n.haz.begin() + (long)pos, std::move(e));
+    } else {
+        n.haz.push_back(std::move(e));
+    }
+}
+
+// QTHazard::constrict (qt_hazard.rs:48-167) for one entry into the 4 quadrants `q`;
+// present[k] = false means QTHazPresence::None.
+static void constrict(const QEntry& e, const float4* q, const HazardRec& hz, QEntry* out, bool* present) {
+    for (int k = 0; k < 4; k++) {
+        out[k] = QEntry{};
+        out[k].slot = e.slot;
+        out[k].version = e.version;
+        present[k] = false;
+    }
+    if (e.presence == PRES_ENTIRE) {
+        for (int k = 0; k < 4; k++) { out[k].presence = PRES_ENTIRE; present[k] = true; }
+        return;
+    }
+    const Polygon& shape = hz.shape;
+    for (int k = 0; k < 4; k++)
+        if (box_relation(shape.bbox, q[k]) == REL_ENCLOSED) {
+            out[k] = e;
+            present[k] = true;
+            return;
+        }
+    bool resolved[4] = {false, false, false, false};
+    for (int k = 0; k < 4; k++) {
+        std::vector<uint32_t> sub;
+        for (uint32_t ei : e.edges)
+            if (box_edge(q[k], shape.edge(ei))) sub.push_back(ei);
+        if (sub.empty()) continue;
+        QEntry& o = out[k];
+        o.presence = PRES_PARTIAL;
+        if (sub.size() == e.edges.size()) {                       // qt_partial_hazard.rs:20-60
+            o.ff = e.ff;
+        } else {
+            float4 b = make_float4(INFINITY, INFINITY, -INFINITY, -INFINITY);
+            for (uint32_t ei : sub) {
+                const float4 ed = shape.edge(ei);
+                b.x = mn(mn(b.x, ed.x), ed.z);
+                b.y = mn(mn(b.y, ed.y), ed.w);
+                b.z = mx(mx(b.z, ed.x), ed.z);
+                b.w = mx(mx(b.w, ed.y), ed.w);
+            }
+            o.ff = (b.x < b.z && b.y < b.w) ? b : e.ff;
+        }
+        o.edges = std::move(sub);
+        present[k] = true;
+        resolved[k] = true;
+    }
+    static const int NB[4][2] = {{1, 3}, {0, 2}, {1, 3}, {0, 2}};   // rect.rs:136
+    for (int k = 0; k < 4; k++) {
+        if (resolved[k]) continue;
+        bool none_nb = false, entire_nb = false;
+        for (int j = 0; j < 2; j++) {
+            const int o = NB[k][j];
+            if (!resolved[o]) continue;
+            if (!present[o]) none_nb = true;
+            else if (out[o].presence == PRES_ENTIRE) entire_nb = true;
+        }
+        bool entire;
+        if (none_nb && !entire_nb) entire = false;
+        else if (entire_nb && !none_nb) entire = true;
+        else {
+            const bool inside = polygon_has_point(shape, (q[k].x + q[k].z) / 2.0f, (q[k].y + q[k].w) / 2.0f);
+            entire = hz.interior() ? inside : !inside;
+        }
+        if (entire) { out[k].presence = PRES_ENTIRE; present[k] = true; }
+        resolved[k] = true;
+    }
+}
+
+void QuadTree::push_to_children(int32_t node, const QEntry& e, const SlotMap& map) {
+    const int32_t base = pool[node].child;
+    float4 q[4];
+    for (int k = 0; k < 4; k++) q[k] = pool[base + k].bbox;
+    QEntry out[4];
+    bool present[4];
+    constrict(e, q, map.slots[e.slot].rec, out, present);
+    for (int k = 0; k < 4; k++)
+        if (present[k]) register_hazard(base + k, std::move(out[k]), map);
+}
+
+// QTNode::register_hazard (qt_node.rs:38-81)
+void QuadTree::register_hazard(int32_t node, QEntry e, const SlotMap& map) {
+    if (pool[node].child < 0 && pool[node].level > 0 && e.presence == PRES_PARTIAL) {
+        alloc_children(node);
+        const size_t n_prev = pool[node].haz.size();
+        for (size_t i = 0; i < n_prev; i++) {
+            QEntry prev = pool[node].haz[i];
+            push_to_children(node, prev, map);
+        }
+    }
+    if (pool[node].child >= 0) push_to_children(node, e, map);
+    add_entry(pool[node], std::move(e));
+}
+
+// QTNode::deregister_hazard (qt_node.rs:83-94)
+void QuadTree::deregister_hazard(int32_t node, uint32_t slot, uint32_t version) {
+    QNode& n = pool[node];
+    size_t i = 0;
+    while (i < n.haz.size() && !(n.haz[i].slot == slot && n.haz[i].version == version)) i++;
+    if (i == n.haz.size()) return;
+    n.n_active_edges -= n.haz[i].presence == PRES_PARTIAL ? n.haz[i].edges.size() : 0;
+    n.haz.erase(n.haz.begin() + (long)i);
+    bool any_partial = false;
+    for (const QEntry& h : n.haz) any_partial |= h.presence == PRES_PARTIAL;
+    if (!any_partial) {
+        free_subtree_children(node);
+    } else if (n.child >= 0) {
+        const int32_t base = n.child;
+        for (int k = 0; k < 4; k++) deregister_hazard(base + k, slot, version);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// BFS flattening
+// ---------------------------------------------------------------------------------------
+void flatten(const QuadTree& qt, const SlotMap& map, FlatCDE& f) {
+    f = FlatCDE{};
+    std::vector<int32_t> order{0};
+    std::vector<uint32_t> up{0};
+    for (size_t i = 0; i < order.size(); i++) {
+        const QNode& n = qt.pool[order[i]];
+        if (n.child >= 0)
+            for (int k = 0; k < 4; k++) {
+                order.push_back(n.child + k);
+                up.push_back((uint32_t)(i << 2) | (uint32_t)k);
+            }
+    }
+    const size_t N = order.size();
+    f.node_bbox.resize(N);
+    f.node_info.resize(N);
+    f.node_up = up;
+    size_t next_child = 1;
+    for (size_t i = 0; i < N; i++) {
+        const QNode& n = qt.pool[order[i]];
+        int4 info;
+        info.x = -1;
+        if (n.child >= 0) {
+            info.x = (int)next_child;
+            next_child += 4;
+        }
+        info.y = (int)f.ent.size();
+        info.z = (int)n.haz.size();
+        info.w = (int)std::min<uint64_t>(n.n_active_edges, 0x7fffffffu);
+        f.node_bbox[i] = n.bbox;
+        f.node_info[i] = info;
+        const bool tests_edges = n.child < 0 || n.n_active_edges <= qt.cd_threshold;
+        for (const QEntry& h : n.haz) {
+            FlatEntry fe{};
+            fe.slot_flags = h.slot | (h.presence == PRES_ENTIRE ? 0x80000000u : 0u);
+            fe.n_edges_total = h.presence == PRES_PARTIAL ? (uint32_t)h.edges.size() : 0u;
+            if (h.presence == PRES_PARTIAL) {
+                fe.ff = h.ff;
+                if (tests_edges) {
+                    fe.edge_begin = (uint32_t)f.edges.size();
+                    fe.edge_count = (uint32_t)h.edges.size();
+                    const Polygon& s = map.slots[h.slot].rec.shape;
+                    for (uint32_t ei : h.edges) f.edges.push_back(s.edge(ei));
+                }
+            }
+            f.ent.push_back(fe);
+        }
+    }
+    const size_t S = map.slots.size();
+    f.haz_bbox.assign(S, make_float4(0, 0, 0, 0));
+    f.haz_poi.assign(S, make_float4(0, 0, 0, 0));
+    f.haz_vr.assign(S, make_uint2(0, 0));
+    for (size_t s = 1; s < S; s++) {
+        if (!map.slots[s].occ) continue;
+        const HazardRec& r = map.slots[s].rec;
+        f.haz_bbox[s] = r.shape.bbox;
+        f.haz_poi[s] = make_float4(r.shape.poi.x, r.shape.poi.y, r.shape.poi.z, r.interior() ? 1.0f : 0.0f);
+        f.haz_vr[s] = make_uint2((uint32_t)f.haz_verts.size(), r.shape.n());
+        f.haz_verts.insert(f.haz_verts.end(), r.shape.v.begin(), r.shape.v.end());
+    }
+}
+
+}  // namespace jg

@@ -1,0 +1,118 @@
+// Host-side data structures of the MI355X collision engine: polygons (SPolygon + surrogate),
+// the hazard slot map, the mutable quadtree (a node pool that keeps the reference's exact
+// structure under register / deregister) and its flattened BFS structure-of-arrays image.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/jagua_amd.h"
+#include "jg_geom.hpp"
+
+namespace jg {
+
+struct Surrogate {
+    std::vector<float4> poles;          // cx, cy, r, 0
+    std::vector<float4> piers;          // x1 y1 x2 y2
+    std::vector<uint32_t> hull_idx;
+    float hull_area = 0.0f;
+    jg_surrogate_config cfg{};
+    bool present = false;
+};
+
+struct Polygon {
+    std::vector<float2> v;
+    float4 bbox{};
+    float area = 0.0f, diameter = 0.0f;
+    float4 poi{};                       // cx, cy, r, 0
+    Surrogate sur;
+    uint32_t n() const { return (uint32_t)v.size(); }
+    float4 edge(uint32_t i) const {
+        uint32_t j = i + 1 == v.size() ? 0 : i + 1;
+        return make_float4(v[i].x, v[i].y, v[j].x, v[j].y);
+    }
+};
+
+// polygon construction (simple_polygon.rs, fail_fast/*.rs, convex_hull.rs)
+bool polygon_new(const float* xy, uint32_t n, Polygon& out, std::string& err);
+bool polygon_generate_surrogate(Polygon& p, const jg_surrogate_config& cfg, std::string& err);
+Polygon polygon_transform(const Polygon& p, const Mat& m);
+bool polygon_has_point(const Polygon& p, float x, float y);
+Mat mat_from_dt(float rotation, float tx, float ty);
+
+// ----------------------------------------------------------------------------- hazards
+struct HazardRec {
+    jg_hazard_entity entity{};
+    Polygon shape;
+    bool dynamic = false;
+    bool interior() const { return entity.kind != JG_ENTITY_EXTERIOR; }
+};
+
+bool entity_eq(const jg_hazard_entity& a, const jg_hazard_entity& b);
+
+// slotmap semantics: slot 0 is a sentinel, freed slots are reused last-in first-out.
+struct SlotMap {
+    struct Slot { uint32_t version = 0; bool occ = false; uint32_t next_free = 0; HazardRec rec; };
+    std::vector<Slot> slots = std::vector<Slot>(1);
+    uint32_t free_head = 1;
+    jg_hazkey insert(HazardRec r);
+    bool valid(jg_hazkey k) const { return k.idx < slots.size() && slots[k.idx].occ && slots[k.idx].version == k.version; }
+    HazardRec remove(jg_hazkey k);
+};
+
+// ----------------------------------------------------------------------------- quadtree
+enum : uint8_t { PRES_PARTIAL = 1, PRES_ENTIRE = 2 };
+
+struct QEntry {                          // QTHazard in one node (qt_hazard.rs:13-33)
+    uint32_t slot = 0, version = 0;
+    uint8_t presence = 0;
+    float4 ff{};                         // ff_bbox of the partial edge subset
+    std::vector<uint32_t> edges;         // edge indices into the hazard polygon
+};
+
+struct QNode {                           // QTNode (qt_node.rs:14-25)
+    uint8_t level = 0;
+    float4 bbox{};
+    int32_t child = -1;                  // pool index of the first of 4 children
+    std::vector<QEntry> haz;             // Entire entries first, then Partial
+    uint64_t n_active_edges = 0;
+};
+
+struct QuadTree {
+    std::vector<QNode> pool;
+    std::vector<int32_t> free_blocks;
+    uint8_t cd_threshold = 16;
+    void init(uint8_t depth, float4 bbox, uint8_t cdt);
+    void register_hazard(int32_t node, QEntry e, const SlotMap& map);
+    void deregister_hazard(int32_t node, uint32_t slot, uint32_t version);
+  private:
+    int32_t alloc_children(int32_t parent);
+    void free_subtree_children(int32_t node);
+    void push_to_children(int32_t node, const QEntry& e, const SlotMap& map);
+};
+
+// Flattened BFS image uploaded to HBM.
+struct FlatEntry {                       // 32 B
+    float4 ff;
+    uint32_t slot_flags;                 // slot | 0x80000000 when Entire
+    uint32_t edge_begin;
+    uint32_t edge_count;
+    uint32_t n_edges_total;              // partial edge count (n_active_edges bookkeeping)
+};
+
+struct FlatCDE {
+    std::vector<float4> node_bbox;
+    std::vector<int4> node_info;         // first_child (BFS, -1 leaf), haz_begin, haz_count, n_active_edges
+    std::vector<uint32_t> node_up;       // parent << 2 | index among siblings
+    std::vector<FlatEntry> ent;
+    std::vector<float4> edges;           // inline hazard edges of entries in leaves / threshold nodes
+    std::vector<float4> haz_bbox;        // per slot
+    std::vector<float4> haz_poi;         // cx, cy, r, interior ? 1 : 0
+    std::vector<uint2> haz_vr;           // vertex offset, count
+    std::vector<float2> haz_verts;
+};
+
+void flatten(const QuadTree& qt, const SlotMap& map, FlatCDE& out);
+
+}  // namespace jg

@@ -1,0 +1,56 @@
+// Definitions of the opaque ABI handles and helpers shared by the library's translation units.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "../../include/jagua_amd.h"
+#include "jg_device.hpp"
+#include "jg_host.hpp"
+
+namespace jg {
+
+struct DevBlob {                       // one device allocation, grown on demand
+    void* ptr = nullptr;
+    size_t cap = 0;
+    DevBlob() = default;
+    DevBlob(const DevBlob&) = delete;
+    DevBlob& operator=(const DevBlob&) = delete;
+    ~DevBlob();
+    jg_status reserve(size_t bytes);
+};
+
+jg_status fail(jg_status code, const std::string& msg);
+jg_status hip_check(hipError_t e, const char* what);
+jg_status device_ready();
+
+}  // namespace jg
+
+struct jg_spolygon {
+    jg::Polygon poly;
+    jg::DevBlob dev;                   // vertices, ff poles, piers
+    jg::ItemView view{};
+    bool dev_valid = false;
+};
+
+struct jg_cde {
+    jg_cde_config cfg{};
+    jg::SlotMap map;
+    jg::QuadTree qt;
+    jg_hazkey exterior{0, 0};
+    bool dirty = true;
+    bool dev_valid = false;
+    jg::DevBlob dev;                   // flattened BFS image
+    jg::DevBlob filter_dev;            // JG_FILTER_KEYS mask
+    jg::CdeView view{};
+    jg_cde_stats stats{};
+};
+
+struct jg_snapshot {
+    std::vector<jg::HazardRec> dynamic;
+};
+
+namespace jg {
+jg_status cde_upload(jg_cde* c);
+jg_status item_upload(jg_spolygon* it);
+jg_status filter_view(jg_cde* c, const jg_hazard_filter* f, FilterView& out);
+}  // namespace jg

@@ -1,0 +1,456 @@
+"""jagua_amd — the collision-query hot path of jagua-rs on one MI355X (gfx950).
+
+Python mirror of the reference API over the C ABI in include/jagua_amd.h:
+
+    SPolygon, SPSurrogateConfig         geometry/primitives/simple_polygon.rs, fail_fast/sp_surrogate.rs
+    DTransformation                     geometry/d_transformation.rs
+    HazardEntity, Hazard, HazKey        collision_detection/hazards/hazard.rs
+    NoFilter, HazKeyFilter, HashedFilter collision_detection/hazards/filter.rs
+    CDEConfig, CDEngine, CDESnapshot    collision_detection/cd_engine.rs
+
+Queries are batched: one call evaluates a whole array of transforms of one item on the GPU
+(lbf samples them one by one in lbf/src/opt/search.rs:41-101).  Transforms and outputs are
+torch tensors on the GPU (device plumbing only) or numpy arrays (copied in and out).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from ._native import JaguaError, check
+
+Q_CLEAR, Q_COLLISION, Q_PANIC_FILTERED_ENTIRE = 0, 1, 2
+QUERY_SURROGATE, QUERY_POLY = 1, 2
+
+ENTITY_PLACED_ITEM, ENTITY_EXTERIOR, ENTITY_HOLE, ENTITY_INFERIOR_QUALITY_ZONE = 0, 1, 2, 3
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _ptr(a, t=C.c_float):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class SPSurrogateConfig:
+    """SPSurrogateConfig (sp_surrogate.rs:120-131); defaults are LBFConfig::default (lbf/src/config.rs:36-40)."""
+
+    def __init__(self, n_pole_limits=((100, 0.0), (20, 0.75), (10, 0.90)), n_ff_poles=2, n_ff_piers=0):
+        self.n_pole_limits = tuple((int(n), float(c)) for n, c in n_pole_limits)
+        self.n_ff_poles = int(n_ff_poles)
+        self.n_ff_piers = int(n_ff_piers)
+
+    def _c(self):
+        c = N.SurrogateConfig()
+        for i, (n, cov) in enumerate(self.n_pole_limits):
+            c.n_pole_limits_n[i] = n
+            c.n_pole_limits_cov[i] = cov
+        c.n_ff_poles = self.n_ff_poles
+        c.n_ff_piers = self.n_ff_piers
+        return c
+
+
+class CDEConfig:
+    """CDEConfig (cd_engine.rs:345-353); defaults from LBFConfig::default (lbf/src/config.rs:33-41)."""
+
+    def __init__(self, quadtree_depth=5, cd_threshold=16, item_surrogate_config=None):
+        self.quadtree_depth = int(quadtree_depth)
+        self.cd_threshold = int(cd_threshold)
+        self.item_surrogate_config = item_surrogate_config or SPSurrogateConfig()
+
+    def _c(self):
+        c = N.CDEConfigC()
+        c.quadtree_depth = self.quadtree_depth
+        c.cd_threshold = self.cd_threshold
+        c.item_surrogate_config = self.item_surrogate_config._c()
+        return c
+
+
+class DTransformation:
+    """DTransformation (d_transformation.rs:11-47): a rotation followed by a translation."""
+
+    def __init__(self, rotation, translation):
+        self.rotation = float(np.float32(rotation))
+        self.translation = (float(np.float32(translation[0])), float(np.float32(translation[1])))
+
+    def as_array(self):
+        return np.array([self.rotation, *self.translation], dtype=np.float32)
+
+    def compose(self):
+        """Transformation matrix rows (m00 m01 m02 m10 m11 m12), transformation.rs:131-139."""
+        return compose(self.as_array()[None])[0]
+
+
+def compose(dt):
+    """DTransformation::compose for an array [n, 3] of (rotation, tx, ty) -> [n, 6] matrices."""
+    dt = _f32(dt).reshape(-1, 3)
+    out = np.empty((dt.shape[0], 6), dtype=np.float32)
+    check(N.lib().jg_transforms_from_dt(_ptr(dt), dt.shape[0], _ptr(out)))
+    return out
+
+
+class SPolygon:
+    """SPolygon (simple_polygon.rs:24-37), optionally carrying its SPSurrogate."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+
+    @classmethod
+    def new(cls, points):
+        """SPolygon::new (simple_polygon.rs:41-71)."""
+        pts = _f32(points).reshape(-1, 2)
+        h = C.c_void_p()
+        check(N.lib().jg_spolygon_new(_ptr(pts), pts.shape[0], C.byref(h)))
+        return cls(h.value)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and N._lib is not None:
+            N._lib.jg_spolygon_free(self._h)
+            self._h = C.c_void_p()
+
+    def generate_surrogate(self, config=None):
+        """SPolygon::generate_surrogate (simple_polygon.rs:73-80)."""
+        cfg = (config or SPSurrogateConfig())._c()
+        check(N.lib().jg_spolygon_generate_surrogate(self._h, C.byref(cfg)))
+        return self
+
+    def transform_clone(self, dt):
+        """transform_clone by dt.compose() (placed_item.rs:27-36)."""
+        d = dt.as_array() if isinstance(dt, DTransformation) else _f32(dt)
+        h = C.c_void_p()
+        check(N.lib().jg_spolygon_transform_clone(self._h, _ptr(d), C.byref(h)))
+        return SPolygon(h.value)
+
+    def info(self):
+        i = N.SPolygonInfoC()
+        check(N.lib().jg_spolygon_get_info(self._h, C.byref(i)))
+        return i
+
+    @property
+    def n_vertices(self):
+        return self.info().n_vertices
+
+    @property
+    def vertices(self):
+        out = np.empty((self.n_vertices, 2), dtype=np.float32)
+        check(N.lib().jg_spolygon_get_vertices(self._h, _ptr(out)))
+        return out
+
+    @property
+    def bbox(self):
+        return np.array(self.info().bbox[:], dtype=np.float32)
+
+    @property
+    def area(self):
+        return np.float32(self.info().area)
+
+    @property
+    def diameter(self):
+        return np.float32(self.info().diameter)
+
+    @property
+    def poi(self):
+        return np.array(self.info().poi[:], dtype=np.float32)
+
+    @property
+    def poles(self):
+        out = np.empty((self.info().n_poles, 3), dtype=np.float32)
+        if len(out):
+            check(N.lib().jg_spolygon_get_poles(self._h, _ptr(out)))
+        return out
+
+    @property
+    def piers(self):
+        out = np.empty((self.info().n_piers, 4), dtype=np.float32)
+        if len(out):
+            check(N.lib().jg_spolygon_get_piers(self._h, _ptr(out)))
+        return out
+
+
+class HazardEntity:
+    """HazardEntity (hazard.rs:38-51)."""
+
+    def __init__(self, kind, a=0, b=0, dt=(0.0, 0.0, 0.0)):
+        self.kind, self.a, self.b = int(kind), int(a), int(b)
+        self.dt = tuple(float(np.float32(x)) for x in dt)
+
+    @classmethod
+    def placed_item(cls, item_id, dt, pk):
+        d = dt.as_array() if isinstance(dt, DTransformation) else _f32(dt)
+        return cls(ENTITY_PLACED_ITEM, item_id, pk, tuple(d))
+
+    @classmethod
+    def exterior(cls):
+        return cls(ENTITY_EXTERIOR)
+
+    @classmethod
+    def hole(cls, idx):
+        return cls(ENTITY_HOLE, idx)
+
+    @classmethod
+    def inferior_quality_zone(cls, quality, idx):
+        return cls(ENTITY_INFERIOR_QUALITY_ZONE, quality, idx)
+
+    def scope_interior(self):
+        """HazardEntity::scope (hazard.rs:55-62)."""
+        return self.kind != ENTITY_EXTERIOR
+
+    def _c(self):
+        e = N.HazardEntityC()
+        e.kind, e.a, e.b = self.kind, self.a, self.b
+        for i in range(3):
+            e.dt[i] = self.dt[i]
+        return e
+
+    def __eq__(self, o):
+        return (self.kind, self.a, self.b, self.dt) == (o.kind, o.a, o.b, o.dt)
+
+    def __hash__(self):
+        return hash((self.kind, self.a, self.b, self.dt))
+
+
+class Hazard:
+    """Hazard (hazard.rs:16-23)."""
+
+    def __init__(self, entity, shape, dynamic):
+        self.entity, self.shape, self.dynamic = entity, shape, bool(dynamic)
+
+
+class HazKey:
+    """HazKey (hazard.rs:8-11); as a filter it deems only itself irrelevant (filter.rs:51-55)."""
+
+    def __init__(self, idx, version):
+        self.idx, self.version = int(idx), int(version)
+
+    def __eq__(self, o):
+        return (self.idx, self.version) == (o.idx, o.version)
+
+    def __hash__(self):
+        return hash((self.idx, self.version))
+
+    def __repr__(self):
+        return f"HazKey({self.idx}v{self.version})"
+
+    def _filter(self):
+        return HazKeyFilter([self])._filter()
+
+
+class NoFilter:
+    """NoFilter (filter.rs:57-65)."""
+
+    def _filter(self):
+        f = N.FilterC()
+        f.kind = 0
+        return f
+
+
+class HazKeyFilter:
+    """HazKeyFilter (filter.rs:13-48): the given keys are irrelevant."""
+
+    def __init__(self, keys):
+        self.keys = list(keys)
+        self._arr = (N.HazKeyC * max(1, len(self.keys)))()
+        for i, k in enumerate(self.keys):
+            self._arr[i].idx, self._arr[i].version = k.idx, k.version
+
+    def _filter(self):
+        f = N.FilterC()
+        f.kind = 1
+        f.n_keys = len(self.keys)
+        f.keys = C.cast(self._arr, C.POINTER(N.HazKeyC))
+        return f
+
+
+class HashedFilter:
+    """A per-query pseudo-random HazardFilter: each query deems a `fraction` of the hazard slots
+    irrelevant, chosen by the hash specified in include/jagua_amd.h."""
+
+    def __init__(self, seed, fraction):
+        self.seed = int(seed) & 0xFFFFFFFF
+        self.threshold = min(0xFFFFFFFF, int(round(float(fraction) * 4294967296.0)))
+
+    def _filter(self):
+        f = N.FilterC()
+        f.kind = 2
+        f.seed, f.threshold = self.seed, self.threshold
+        return f
+
+
+class CDESnapshot:
+    """CDESnapshot (cd_engine.rs:356-359)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and N._lib is not None:
+            N._lib.jg_snapshot_free(self._h)
+            self._h = C.c_void_p()
+
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def _stream_ptr(stream):
+    if stream is not None:
+        return C.c_void_p(stream)
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class CDEngine:
+    """CDEngine (cd_engine.rs:21-342) with its quadtree flattened into HBM."""
+
+    def __init__(self, bbox, static_hazards, config=None):
+        self.config = config or CDEConfig()
+        bb = _f32(bbox).reshape(4)
+        n = len(static_hazards)
+        ents = (N.HazardEntityC * max(1, n))()
+        shapes = (C.c_void_p * max(1, n))()
+        dyn = (C.c_uint8 * max(1, n))()
+        for i, h in enumerate(static_hazards):
+            ents[i] = h.entity._c()
+            shapes[i] = h.shape._h.value
+            dyn[i] = 1 if h.dynamic else 0
+        self._h = C.c_void_p()
+        cfg = self.config._c()
+        check(N.lib().jg_cde_new(_ptr(bb), C.byref(cfg), ents, shapes, dyn, n, C.byref(self._h)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and N._lib is not None:
+            N._lib.jg_cde_free(self._h)
+            self._h = C.c_void_p()
+
+    def register_hazard(self, hazard):
+        """CDEngine::register_hazard (cd_engine.rs:59-69); returns the new HazKey."""
+        k = N.HazKeyC()
+        e = hazard.entity._c()
+        check(N.lib().jg_cde_register_hazard(self._h, C.byref(e), hazard.shape._h, 1 if hazard.dynamic else 0, C.byref(k)))
+        return HazKey(k.idx, k.version)
+
+    def deregister_hazard_by_entity(self, entity):
+        e = entity._c()
+        check(N.lib().jg_cde_deregister_hazard_by_entity(self._h, C.byref(e)))
+
+    def deregister_hazard_by_key(self, key):
+        check(N.lib().jg_cde_deregister_hazard_by_key(self._h, N.HazKeyC(key.idx, key.version)))
+
+    def save(self):
+        h = C.c_void_p()
+        check(N.lib().jg_cde_save(self._h, C.byref(h)))
+        return CDESnapshot(h.value)
+
+    def restore(self, snapshot):
+        check(N.lib().jg_cde_restore(self._h, snapshot._h))
+
+    def bbox(self):
+        out = np.empty(4, dtype=np.float32)
+        check(N.lib().jg_cde_bbox(self._h, _ptr(out)))
+        return out
+
+    @property
+    def hkey_exterior(self):
+        k = N.HazKeyC()
+        check(N.lib().jg_cde_hkey_exterior(self._h, C.byref(k)))
+        return HazKey(k.idx, k.version)
+
+    def sync(self):
+        check(N.lib().jg_cde_sync(self._h))
+
+    def stats(self):
+        s = N.CDEStatsC()
+        check(N.lib().jg_cde_get_stats(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in N.CDEStatsC._fields_}
+
+    def dump(self):
+        """Flattened tree records (node [n,8] u32, hazard [m,8] u32), BFS order."""
+        nn, nh = C.c_uint64(), C.c_uint64()
+        check(N.lib().jg_cde_dump(self._h, None, 0, None, 0, C.byref(nn), C.byref(nh)))
+        nodes = np.zeros((nn.value, 8), dtype=np.uint32)
+        haz = np.zeros((max(1, nh.value), 8), dtype=np.uint32)
+        check(N.lib().jg_cde_dump(self._h, _ptr(nodes, C.c_uint32), nn.value, _ptr(haz, C.c_uint32), nh.value,
+                                  C.byref(nn), C.byref(nh)))
+        return nodes, haz[: nh.value]
+
+    # -- queries ------------------------------------------------------------------------
+    def detect_batch(self, item, mats, filter=None, flags=0, out=None, stream=None):
+        """status[i] = detect_surrogate_collision || detect_poly_collision for transform i
+        (ci_bench.rs:107-120).  mats: [n, 6] f32, a torch GPU tensor or a numpy array."""
+        f = (filter or NoFilter())._filter()
+        if _is_torch(mats):
+            import torch
+            n = mats.shape[0]
+            if out is None:
+                out = torch.empty(n, dtype=torch.uint8, device=mats.device)
+            check(N.lib().jg_detect_batch(self._h, item._h, C.c_void_p(mats.data_ptr()), n, C.byref(f), flags,
+                                          C.c_void_p(out.data_ptr()), _stream_ptr(stream)))
+            return out
+        m = _f32(mats).reshape(-1, 6)
+        st = np.empty(m.shape[0], dtype=np.uint8)
+        check(N.lib().jg_detect_batch_host(self._h, item._h, _ptr(m), m.shape[0], C.byref(f), flags,
+                                           _ptr(st, C.c_uint8)))
+        return st
+
+    def collect_batch(self, item, mats, capacity=16, stream=None):
+        """Per transform i: (set size, set hash, first `capacity` slot keys ascending) of
+        collect_surrogate_collisions ∪ collect_poly_collisions (ci_bench.rs:56-69)."""
+        if _is_torch(mats):
+            import torch
+            n = mats.shape[0]
+            cnt = torch.empty(n, dtype=torch.int32, device=mats.device)
+            hsh = torch.empty(n, dtype=torch.int64, device=mats.device)
+            keys = torch.empty((n, max(1, capacity)), dtype=torch.int32, device=mats.device)
+            check(N.lib().jg_collect_batch(self._h, item._h, C.c_void_p(mats.data_ptr()), n, capacity,
+                                           C.c_void_p(cnt.data_ptr()), C.c_void_p(hsh.data_ptr()),
+                                           C.c_void_p(keys.data_ptr()) if capacity else None, _stream_ptr(stream)))
+            return cnt, hsh, keys
+        m = _f32(mats).reshape(-1, 6)
+        n = m.shape[0]
+        cnt = np.empty(n, dtype=np.uint32)
+        hsh = np.empty(n, dtype=np.uint64)
+        keys = np.empty((n, max(1, capacity)), dtype=np.uint32)
+        check(N.lib().jg_collect_batch_host(self._h, item._h, _ptr(m), n, capacity, _ptr(cnt, C.c_uint32),
+                                            _ptr(hsh, C.c_uint64), _ptr(keys, C.c_uint32) if capacity else None))
+        return cnt, hsh, keys
+
+
+class QueryPlan:
+    """Batched persistent form: many (engine, item, transform range) sets in one launch."""
+
+    def __init__(self, sets, chunk=0, workgroups=0, lanes_per_query=0, flags=0):
+        n = len(sets)
+        arr = (N.QuerySetC * n)()
+        self._keep = []
+        for i, (cde, item, first, count, filt) in enumerate(sets):
+            arr[i].cde = cde._h.value
+            arr[i].item = item._h.value
+            arr[i].first = first
+            arr[i].count = count
+            f = (filt or NoFilter())
+            self._keep.append(f)
+            arr[i].filter = f._filter()
+        cfg = N.PlanConfigC(chunk, workgroups, lanes_per_query, flags)
+        self._h = C.c_void_p()
+        self._refs = [s[0] for s in sets] + [s[1] for s in sets]
+        check(N.lib().jg_plan_create(arr, n, C.byref(cfg), C.byref(self._h)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and N._lib is not None:
+            N._lib.jg_plan_free(self._h)
+            self._h = C.c_void_p()
+
+    def detect(self, mats, out, stream=None):
+        check(N.lib().jg_plan_detect(self._h, C.c_void_p(mats.data_ptr()), C.c_void_p(out.data_ptr()),
+                                     _stream_ptr(stream)))
+        return out
+
+    def detect_graph(self, mats, out, replays=1, stream=None):
+        check(N.lib().jg_plan_detect_graph(self._h, C.c_void_p(mats.data_ptr()), C.c_void_p(out.data_ptr()),
+                                           replays, _stream_ptr(stream)))
+        return out
+
+
+def version():
+    return N.lib().jg_version().decode()

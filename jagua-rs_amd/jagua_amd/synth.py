@@ -1,0 +1,155 @@
+"""Synthetic nesting workloads (libjagua_synth.so) and their assembly into CDEngines.
+
+A Scene is a container (plus holes) and placed items given as local polygons with their
+DTransformation; `build_cde` registers it exactly as jagua-rs builds a layout:
+Container::new (entities/container.rs:57-70: Exterior then holes, on the square-inflated
+container bbox), then Layout::place_item per item (entities/layout.rs:73-85).
+"""
+import concurrent.futures as cf
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+from . import (CDEConfig, CDEngine, Hazard, HazardEntity, SPSurrogateConfig, SPolygon)
+
+F32 = np.float32
+
+
+@dataclass
+class Scene:
+    container: np.ndarray
+    holes: list = field(default_factory=list)
+    items: list = field(default_factory=list)       # (local vertices [n,2] f32, dt [3] f32)
+    depth: int = 5
+    cd_threshold: int = 16
+
+    def cde_bbox(self):
+        """Container::new: outer.bbox.inflate_to_square() (container.rs:68, rect.rs:93-109), in f32."""
+        c = self.container.astype(F32)
+        x0, y0 = F32(c[:, 0].min()), F32(c[:, 1].min())
+        x1, y1 = F32(c[:, 0].max()), F32(c[:, 1].max())
+        w, h = F32(x1 - x0), F32(y1 - y0)
+        dx = dy = F32(0.0)
+        if h < w:
+            dy = F32(F32(w - h) / F32(2.0))
+        elif w < h:
+            dx = F32(F32(h - w) / F32(2.0))
+        return np.array([F32(x0 - dx), F32(y0 - dy), F32(x1 + dx), F32(y1 + dy)], dtype=F32)
+
+    @property
+    def n_edges(self):
+        return sum(len(v) for v, _ in self.items) + len(self.container) + sum(len(h) for h in self.holes)
+
+
+def layout_params(**kw):
+    p = N.LayoutParams()
+    defaults = dict(seed=0, container_kind=0, cw=1000.0, ch=400.0, container_nv_lo=4, container_nv_hi=4, n_holes=0,
+                    hole_kind=0, hole_lo=50.0, hole_hi=300.0, hole_nv_lo=8, hole_nv_hi=64, n_items=40,
+                    item_nv_lo=12, item_nv_hi=12, item_r_lo=20.0, item_r_hi=40.0, item_shape=0, min_sep=0.0,
+                    rot_kind=0, placement=0, max_attempts=20000, lbf_samples=2000)
+    defaults.update(kw)
+    for k, v in defaults.items():
+        setattr(p, k, v)
+    return p
+
+
+def generate_scene(depth=5, cd_threshold=16, **kw):
+    L = N.synth()
+    p = layout_params(**kw)
+    h = L.js_layout_generate(C.byref(p))
+    try:
+        sz = (C.c_uint32 * 3)()
+        L.js_layout_sizes(h, sz)
+        cont = np.empty((sz[0], 2), dtype=F32)
+        L.js_layout_container(h, cont.ctypes.data_as(N.c_f32p))
+        holes = []
+        for i in range(sz[1]):
+            a = np.empty((L.js_layout_hole_n(h, i), 2), dtype=F32)
+            L.js_layout_hole(h, i, a.ctypes.data_as(N.c_f32p))
+            holes.append(a)
+        items = []
+        for i in range(sz[2]):
+            a = np.empty((L.js_layout_item_n(h, i), 2), dtype=F32)
+            dt = np.empty(3, dtype=F32)
+            L.js_layout_item(h, i, a.ctypes.data_as(N.c_f32p), dt.ctypes.data_as(N.c_f32p))
+            items.append((a, dt))
+    finally:
+        L.js_layout_free(h)
+    return Scene(cont, holes, items, depth, cd_threshold)
+
+
+def polygon(seed, nv_lo, nv_hi, r_lo, r_hi, shape=0):
+    xy = np.empty((nv_hi, 2), dtype=F32)
+    n = N.synth().js_polygon(seed, nv_lo, nv_hi, r_lo, r_hi, shape, xy.ctypes.data_as(N.c_f32p))
+    return xy[:n].copy()
+
+
+def transforms(seed, n, box, rot_kind=0):
+    """[n, 3] DTransformations: (rotation, tx, ty) uniform in box; rot_kind 1 = multiples of 45 deg."""
+    dt = np.empty((n, 3), dtype=F32)
+    N.synth().js_transforms(seed, n, float(box[0]), float(box[1]), float(box[2]), float(box[3]), rot_kind,
+                            dt.ctypes.data_as(N.c_f32p))
+    return dt
+
+
+def _threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def build_cde(scene, surrogate_config=None):
+    """The product CDEngine of a scene (Exterior, holes, then placed items in order)."""
+    cfg = CDEConfig(scene.depth, scene.cd_threshold, surrogate_config or SPSurrogateConfig())
+    cont = SPolygon.new(scene.container)
+    haz = [Hazard(HazardEntity.exterior(), cont, False)]
+    haz += [Hazard(HazardEntity.hole(i), SPolygon.new(h), False) for i, h in enumerate(scene.holes)]
+    cde = CDEngine(scene.cde_bbox(), haz, cfg)
+    with cf.ThreadPoolExecutor(_threads()) as ex:
+        shapes = list(ex.map(lambda it: SPolygon.new(it[0]).transform_clone(it[1]), scene.items))
+    keys = []
+    for i, ((_, dt), s) in enumerate(zip(scene.items, shapes)):
+        keys.append(cde.register_hazard(Hazard(HazardEntity.placed_item(i, dt, i), s, True)))
+    return cde, keys
+
+
+def make_items(polys, surrogate_config=None):
+    """Query items: SPolygon::new + generate_surrogate for each local polygon (threaded)."""
+    cfg = surrogate_config or SPSurrogateConfig()
+    with cf.ThreadPoolExecutor(_threads()) as ex:
+        return list(ex.map(lambda p: SPolygon.new(p).generate_surrogate(cfg), polys))
+
+
+# ---------------------------------------------------------------------------- workloads
+# Configurations of the collision-query benchmark (sizes in the docstrings).
+
+def config1_scene(seed=0):
+    """1000x400 rectangle, 40 twelve-vertex stars (radii U[20,40]) placed left-bottom-fill
+    with rotations in multiples of 45 degrees; quadtree depth 5."""
+    return generate_scene(depth=5, seed=seed, container_kind=0, cw=1000.0, ch=400.0, n_items=40, item_nv_lo=12,
+                          item_nv_hi=12, item_r_lo=20.0, item_r_hi=40.0, item_shape=0, rot_kind=1, placement=1,
+                          lbf_samples=4000, min_sep=0.0)
+
+
+def config1_queries(seed=0, n=4096):
+    """One 16-vertex star (radii U[15,35]) at n transforms over the container, rotations k*45 deg."""
+    q = polygon(seed * 7919 + 11, 16, 16, 15.0, 35.0, 0)
+    return [q], transforms(seed * 7919 + 12, n, (0.0, 0.0, 1000.0, 400.0), rot_kind=1)
+
+
+def config2_scene(seed=2, n_items=2000):
+    """4000x1500 rectangle with 5 rectangular holes (sides U[50,300]); n_items star or concave
+    polygons (radius 10-30), log-uniform 8-256 vertices, minimum separation 0.5; quadtree depth 8."""
+    return generate_scene(depth=8, seed=seed, container_kind=0, cw=4000.0, ch=1500.0, n_holes=5, hole_kind=0,
+                          hole_lo=50.0, hole_hi=300.0, n_items=n_items, item_nv_lo=8, item_nv_hi=256,
+                          item_r_lo=10.0, item_r_hi=30.0, item_shape=2, min_sep=0.5, rot_kind=0, placement=0,
+                          max_attempts=20000)
+
+
+def config2_queries(seed=2, n_items=64, n_per_item=16384):
+    """n_items query polygons (log-uniform 8-512 vertices) with n_per_item continuous-rotation
+    transforms each over the container bbox."""
+    polys = [polygon(seed * 100003 + i, 8, 512, 10.0, 30.0, 2) for i in range(n_items)]
+    dts = [transforms(seed * 100003 + 7 * i + 1, n_per_item, (0.0, 0.0, 4000.0, 1500.0), 0) for i in range(n_items)]
+    return polys, dts
