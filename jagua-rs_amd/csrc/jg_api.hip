@@ -166,6 +166,7 @@ jg_status cde_upload(jg_cde* c) {
     c->stats.upload_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
     c->dirty = false;
     c->dev_valid = true;
+    c->generation++;
     return JG_OK;
 }
 

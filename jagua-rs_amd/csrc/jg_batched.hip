@@ -1,18 +1,383 @@
-// Batched persistent query form (jg_plan_*).
+// Batched persistent query form (jg_plan_*): many (engine, item, transform range) sets run
+// in one launch.  Work units of up to 64 queries are pulled from a device queue by
+// wavefronts:
+//   1. surrogate fail-fast (cd_engine.rs:204-223): lanes = (query, ff pole/pier) pairs, each
+//      lane descends from the root; a ballot yields the chunk's surviving queries;
+//   2. polygon query (cd_engine.rs:160-197) per survivor: lanes = query edges, descending from
+//      the virtual root, with a wave-wide early exit after every 64 edges;
+//   3. containment (cd_engine.rs:176-193): lanes = the virtual root's hazard entries.
+// Units are ordered by estimated cost (largest items first) so the tail stays short.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <vector>
 
 #include "../../include/jagua_amd.h"
 #include "jg_internal.hpp"
 
 using namespace jg;
 
-extern "C" jg_status jg_plan_create(const jg_query_set*, uint32_t, const jg_plan_config*, jg_plan**) {
-    return fail(JG_ERR_INTERNAL, "batched plans are not built yet");
+namespace {
+
+struct SetDesc {
+    CdeView cde;
+    ItemView item;
+    FilterView filter;
+    uint64_t first;
+    uint32_t flags, pad;
+};
+
+struct Unit {             // 16 B
+    uint32_t set;
+    uint32_t q0;          // first query, relative to the set
+    uint32_t count;       // <= 64
+    uint32_t pad;
+};
+
+constexpr int WAVE = 64;
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ float wmin(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
 }
-extern "C" void jg_plan_free(jg_plan*) {}
-extern "C" jg_status jg_plan_detect(jg_plan*, const float*, uint8_t*, void*) {
-    return fail(JG_ERR_INTERNAL, "batched plans are not built yet");
+__device__ __forceinline__ float wmax(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
 }
-extern "C" jg_status jg_plan_detect_graph(jg_plan*, const float*, uint8_t*, uint32_t, void*) {
-    return fail(JG_ERR_INTERNAL, "batched plans are not built yet");
+
+}  // namespace
+
+__global__ void k_reset_counter(uint32_t* counter) {
+    if (threadIdx.x == 0) *counter = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_detect_batched(const SetDesc* __restrict__ sets, const Unit* __restrict__ units,
+                                                        uint32_t n_units, uint32_t* __restrict__ counter,
+                                                        const float* __restrict__ m6, uint8_t* __restrict__ status) {
+    const uint32_t lane = threadIdx.x & 63u;
+    for (;;) {
+        uint32_t u = 0;
+        if (lane == 0) u = atomicAdd(counter, 1u);
+        u = (uint32_t)__shfl((int)u, 0, 64);
+        if (u >= n_units) return;
+        const Unit un = units[u];
+        const SetDesc& sd = sets[un.set];
+        const CdeView c = sd.cde;
+        const ItemView it = sd.item;
+        const FilterView f = sd.filter;
+        const uint32_t flags = sd.flags;
+        const uint64_t base = sd.first;
+        const uint32_t cnt = un.count;
+
+        // ---- phase 1: surrogate, lanes = (query, entity)
+        uint64_t hit_mask = 0;                        // bit i: query q0+i collided in the surrogate
+        const uint32_t n_ent = (flags & JG_QUERY_SURROGATE) ? it.n_poles + it.n_piers : 0u;
+        if (n_ent > 0) {
+            uint32_t g = 1;
+            while (g < n_ent && g < 64u) g <<= 1;     // lanes per query
+            const uint32_t qpi = 64u / g;             // queries per pass
+            for (uint32_t b = 0; b < cnt; b += qpi) {
+                const uint32_t qi = b + lane / g;
+                bool hit = false;
+                if (qi < cnt) {
+                    const uint64_t q = (uint64_t)un.q0 + qi;
+                    const Mat m = load_mat(m6, base + q);
+                    for (uint32_t e = lane % g; e < n_ent && !hit; e += g) {
+                        const float4 p = it.ff[e];
+                        if (e < it.n_poles) {
+                            const float2 cc = xform(m, p.x, p.y);
+                            hit = collides_circle(c, 0, cc.x, cc.y, p.z, f, q);
+                        } else {
+                            const float2 a = xform(m, p.x, p.y), bb = xform(m, p.z, p.w);
+                            hit = collides_edge(c, 0, make_float4(a.x, a.y, bb.x, bb.y), f, q);
+                        }
+                    }
+                }
+                const uint64_t bal = ballot(hit);
+                // fold each group of g lanes into one bit per query
+                for (uint32_t j = 0; j < qpi && b + j < cnt; j++) {
+                    const uint64_t grp = g == 64 ? ~0ull : (((1ull << g) - 1ull) << (j * g));
+                    if (bal & grp) hit_mask |= 1ull << (b + j);
+                }
+            }
+        }
+
+        // ---- phase 2 + 3: polygon query per surviving query, lanes = edges
+        uint8_t my_status = JG_Q_COLLISION;           // lane l reports query q0 + l
+        uint64_t todo = (cnt == 64 ? ~0ull : ((1ull << cnt) - 1ull)) & ~hit_mask;
+        if (!(flags & JG_QUERY_POLY)) {
+            my_status = (hit_mask >> lane) & 1ull ? JG_Q_COLLISION : JG_Q_CLEAR;
+            todo = 0;
+        }
+        while (todo) {
+            const uint32_t qi = (uint32_t)__builtin_ctzll(todo);
+            todo &= todo - 1;
+            const uint64_t q = (uint64_t)un.q0 + qi;
+            const Mat m = load_mat(m6, base + q);
+            float4 bb = make_float4(3.40282347e+38f, 3.40282347e+38f, -3.40282347e+38f, -3.40282347e+38f);
+            for (uint32_t i = lane; i < it.n_v; i += WAVE) {
+                const float2 p = xform(m, it.v[i].x, it.v[i].y);
+                bb.x = mn(bb.x, p.x); bb.y = mn(bb.y, p.y); bb.z = mx(bb.z, p.x); bb.w = mx(bb.w, p.y);
+            }
+            const float4 qbb = make_float4(wmin(bb.x), wmin(bb.y), wmax(bb.z), wmax(bb.w));
+            uint8_t res;
+            if (box_relation(c.node_bbox[0], qbb) != REL_SURROUNDING) {
+                res = JG_Q_COLLISION;
+            } else {
+                const int vr = virtual_root(c, qbb);
+                bool any = false;
+                for (uint32_t i0 = 0; i0 < it.n_v && !any; i0 += WAVE) {
+                    const uint32_t i = i0 + lane;
+                    bool hit = false;
+                    if (i < it.n_v) {
+                        const uint32_t j = i + 1 == it.n_v ? 0 : i + 1;
+                        const float2 a = xform(m, it.v[i].x, it.v[i].y);
+                        const float2 b2 = xform(m, it.v[j].x, it.v[j].y);
+                        hit = collides_edge(c, vr, make_float4(a.x, a.y, b2.x, b2.y), f, q);
+                    }
+                    any = ballot(hit) != 0ull;
+                }
+                if (any) {
+                    res = JG_Q_COLLISION;
+                } else {
+                    const int4 info = c.node_info[vr];
+                    if (info.z > 0 && (c.ent[info.y].slot_flags & ENTIRE_BIT)) {
+                        res = JG_Q_PANIC_FILTERED_ENTIRE;
+                    } else {
+                        const float2 qpoi = xform(m, it.poi.x, it.poi.y);
+                        bool hit = false;
+                        for (int i = info.y + (int)lane; i < info.y + info.z; i += WAVE) {
+                            const uint32_t slot = c.ent[i].slot_flags & SLOT_MASK;
+                            if (irrelevant(f, q, slot)) continue;
+                            if (containment(c, slot, it, m, qbb, qpoi)) { hit = true; break; }
+                        }
+                        res = ballot(hit) != 0ull ? JG_Q_COLLISION : JG_Q_CLEAR;
+                    }
+                }
+            }
+            if (lane == qi) my_status = res;
+        }
+        if (lane < cnt) status[base + un.q0 + lane] = my_status;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+struct jg_plan {
+    std::vector<jg_query_set> sets;
+    std::vector<jg_hazard_filter> filters;
+    std::vector<std::vector<jg_hazkey>> filter_keys;
+    jg_plan_config cfg{};
+    DevBlob table;                  // SetDesc[] then Unit[] then the counter
+    DevBlob masks;                  // JG_FILTER_KEYS masks of every set
+    std::vector<uint64_t> generations;
+    uint32_t n_units = 0;
+    size_t off_units = 0, off_counter = 0;
+    uint32_t grid = 0;
+    // graph cache
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    const float* g_m6 = nullptr;
+    uint8_t* g_status = nullptr;
+    hipStream_t g_stream = nullptr;
+    ~jg_plan() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+namespace {
+
+jg_status plan_build_table(jg_plan* p) {
+    // engines and items resident and current
+    for (auto& s : p->sets) {
+        jg_status st = cde_upload(s.cde);
+        if (st != JG_OK) return st;
+        st = item_upload(const_cast<jg_spolygon*>(s.item));
+        if (st != JG_OK) return st;
+    }
+    std::vector<uint64_t> gens;
+    for (auto& s : p->sets) gens.push_back(s.cde->generation);
+    if (gens == p->generations && p->table.ptr) return JG_OK;
+
+    // filter masks
+    std::vector<uint32_t> mask_words;
+    std::vector<size_t> mask_off(p->sets.size(), 0);
+    for (size_t i = 0; i < p->sets.size(); i++) {
+        const jg_hazard_filter& f = p->filters[i];
+        if (f.kind != JG_FILTER_KEYS) continue;
+        jg_cde* c = p->sets[i].cde;
+        const uint32_t words = (uint32_t)((c->map.slots.size() + 31) / 32);
+        mask_off[i] = mask_words.size();
+        mask_words.resize(mask_words.size() + words, 0u);
+        for (const jg_hazkey& k : p->filter_keys[i])
+            if (c->map.valid(k)) mask_words[mask_off[i] + (k.idx >> 5)] |= 1u << (k.idx & 31u);
+    }
+    if (!mask_words.empty()) {
+        jg_status st = p->masks.reserve(mask_words.size() * 4);
+        if (st != JG_OK) return st;
+        jg_status e = hip_check(hipMemcpy(p->masks.ptr, mask_words.data(), mask_words.size() * 4, hipMemcpyHostToDevice),
+                                "hipMemcpy masks");
+        if (e != JG_OK) return e;
+    }
+    std::vector<SetDesc> sd(p->sets.size());
+    for (size_t i = 0; i < p->sets.size(); i++) {
+        const jg_query_set& s = p->sets[i];
+        sd[i].cde = s.cde->view;
+        sd[i].item = s.item->view;
+        sd[i].flags = p->cfg.flags ? p->cfg.flags : (JG_QUERY_SURROGATE | JG_QUERY_POLY);
+        if (!(sd[i].flags & JG_QUERY_SURROGATE)) { sd[i].item.n_poles = 0; sd[i].item.n_piers = 0; }
+        sd[i].first = s.first;
+        const jg_hazard_filter& f = p->filters[i];
+        sd[i].filter = FilterView{nullptr, 0, 0, 0, 0};
+        if (f.kind == JG_FILTER_HASHED) {
+            sd[i].filter.kind = 2;
+            sd[i].filter.seed = f.seed;
+            sd[i].filter.threshold = f.threshold;
+        } else if (f.kind == JG_FILTER_KEYS) {
+            sd[i].filter.kind = 1;
+            sd[i].filter.mask = (const uint32_t*)p->masks.ptr + mask_off[i];
+            sd[i].filter.n_words = (uint32_t)((s.cde->map.slots.size() + 31) / 32);
+        }
+    }
+    // work units: sets in decreasing estimated cost, chunks of `chunk` queries
+    const uint32_t chunk = p->cfg.chunk ? std::min<uint32_t>(p->cfg.chunk, 64u) : 64u;
+    std::vector<size_t> order(p->sets.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+        return p->sets[a].item->poly.n() > p->sets[b].item->poly.n();
+    });
+    std::vector<Unit> units;
+    for (size_t oi : order) {
+        const jg_query_set& s = p->sets[oi];
+        for (uint64_t q = 0; q < s.count; q += chunk)
+            units.push_back(Unit{(uint32_t)oi, (uint32_t)q, (uint32_t)std::min<uint64_t>(chunk, s.count - q), 0u});
+    }
+    p->n_units = (uint32_t)units.size();
+    const size_t o_units = (sizeof(SetDesc) * sd.size() + 255) & ~(size_t)255;
+    const size_t o_ctr = (o_units + sizeof(Unit) * units.size() + 255) & ~(size_t)255;
+    const size_t total = o_ctr + 256;
+    std::vector<uint8_t> host(total, 0);
+    std::memcpy(host.data(), sd.data(), sizeof(SetDesc) * sd.size());
+    std::memcpy(host.data() + o_units, units.data(), sizeof(Unit) * units.size());
+    jg_status st = p->table.reserve(total);
+    if (st != JG_OK) return st;
+    st = hip_check(hipMemcpy(p->table.ptr, host.data(), total, hipMemcpyHostToDevice), "hipMemcpy plan");
+    if (st != JG_OK) return st;
+    p->off_units = o_units;
+    p->off_counter = o_ctr;
+    p->generations = gens;
+    if (p->gexec) {                 // a captured graph refers to the old table
+        (void)hipGraphExecDestroy(p->gexec);
+        (void)hipGraphDestroy(p->graph);
+        p->gexec = nullptr;
+        p->graph = nullptr;
+    }
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const uint32_t want = p->cfg.workgroups ? p->cfg.workgroups : (uint32_t)cus * 8u;
+    p->grid = std::max<uint32_t>(1u, std::min<uint32_t>(want, (p->n_units + 3u) / 4u));
+    return JG_OK;
+}
+
+jg_status plan_launch(jg_plan* p, const float* d_m6, uint8_t* d_status, hipStream_t stream) {
+    uint8_t* base = (uint8_t*)p->table.ptr;
+    uint32_t* counter = (uint32_t*)(base + p->off_counter);
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, stream, counter);
+    hipLaunchKernelGGL(k_detect_batched, dim3(p->grid), dim3(256), 0, stream, (const SetDesc*)base,
+                       (const Unit*)(base + p->off_units), p->n_units, counter, d_m6, d_status);
+    return hip_check(hipGetLastError(), "k_detect_batched launch");
+}
+
+}  // namespace
+
+extern "C" jg_status jg_plan_create(const jg_query_set* sets, uint32_t n_sets, const jg_plan_config* cfg, jg_plan** out) {
+    if (!out || (n_sets && !sets)) return fail(JG_ERR_INVALID_ARG, "null argument");
+    auto p = std::make_unique<jg_plan>();
+    if (cfg) p->cfg = *cfg;
+    const uint32_t flags = p->cfg.flags ? p->cfg.flags : (JG_QUERY_SURROGATE | JG_QUERY_POLY);
+    for (uint32_t i = 0; i < n_sets; i++) {
+        const jg_query_set& s = sets[i];
+        if (!s.cde || !s.item) return fail(JG_ERR_INVALID_ARG, "null engine or item in a query set");
+        if (s.count > 0xffffffffull) return fail(JG_ERR_INVALID_ARG, "query set too large");
+        const Polygon& poly = s.item->poly;
+        if (flags & JG_QUERY_SURROGATE) {
+            if (!poly.sur.present) return fail(JG_ERR_SURROGATE, "surrogate not generated");
+            if (poly.sur.cfg.n_ff_poles > poly.sur.poles.size()) return fail(JG_ERR_SURROGATE, "ff_poles slice out of range");
+        }
+        if (s.filter.kind > JG_FILTER_HASHED) return fail(JG_ERR_INVALID_ARG, "unknown filter kind");
+        p->sets.push_back(s);
+        p->filters.push_back(s.filter);
+        std::vector<jg_hazkey> keys;
+        if (s.filter.kind == JG_FILTER_KEYS && s.filter.n_keys) keys.assign(s.filter.keys, s.filter.keys + s.filter.n_keys);
+        p->filter_keys.push_back(std::move(keys));
+        p->filters.back().keys = nullptr;
+    }
+    jg_status st = device_ready();
+    if (st != JG_OK) return st;
+    st = plan_build_table(p.get());
+    if (st != JG_OK) return st;
+    *out = p.release();
+    return JG_OK;
+}
+
+extern "C" void jg_plan_free(jg_plan* p) { delete p; }
+
+extern "C" jg_status jg_plan_detect(jg_plan* p, const float* d_m6, uint8_t* d_status, void* stream) {
+    if (!p) return fail(JG_ERR_INVALID_ARG, "null plan");
+    if (p->n_units == 0) return JG_OK;
+    if (!d_m6 || !d_status) return fail(JG_ERR_INVALID_ARG, "null device buffer");
+    jg_status st = plan_build_table(p);
+    if (st != JG_OK) return st;
+    return plan_launch(p, d_m6, d_status, (hipStream_t)stream);
+}
+
+extern "C" jg_status jg_plan_detect_graph(jg_plan* p, const float* d_m6, uint8_t* d_status, uint32_t replays, void* stream) {
+    if (!p) return fail(JG_ERR_INVALID_ARG, "null plan");
+    if (p->n_units == 0 || replays == 0) return JG_OK;
+    if (!d_m6 || !d_status) return fail(JG_ERR_INVALID_ARG, "null device buffer");
+    jg_status st = plan_build_table(p);
+    if (st != JG_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    if (!p->gexec || p->g_m6 != d_m6 || p->g_status != d_status || p->g_stream != s) {
+        if (p->gexec) {
+            (void)hipGraphExecDestroy(p->gexec);
+            (void)hipGraphDestroy(p->graph);
+            p->gexec = nullptr;
+            p->graph = nullptr;
+        }
+        hipStream_t cap = nullptr;
+        st = hip_check(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreate");
+        if (st != JG_OK) return st;
+        st = hip_check(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        if (st == JG_OK) {
+            jg_status ls = plan_launch(p, d_m6, d_status, cap);
+            hipGraph_t g = nullptr;
+            jg_status es = hip_check(hipStreamEndCapture(cap, &g), "hipStreamEndCapture");
+            st = ls != JG_OK ? ls : es;
+            if (st == JG_OK) {
+                p->graph = g;
+                st = hip_check(hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+            } else if (g) {
+                (void)hipGraphDestroy(g);
+            }
+        }
+        (void)hipStreamDestroy(cap);
+        if (st != JG_OK) return st;
+        p->g_m6 = d_m6;
+        p->g_status = d_status;
+        p->g_stream = s;
+    }
+    for (uint32_t r = 0; r < replays; r++) {
+        st = hip_check(hipGraphLaunch(p->gexec, s), "hipGraphLaunch");
+        if (st != JG_OK) return st;
+    }
+    return JG_OK;
 }

@@ -43,6 +43,7 @@ struct jg_cde {
     jg::DevBlob filter_dev;            // JG_FILTER_KEYS mask
     jg::CdeView view{};
     jg_cde_stats stats{};
+    uint64_t generation = 0;           // bumped on every upload of a new image
 };
 
 struct jg_snapshot {

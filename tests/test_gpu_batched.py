@@ -1,0 +1,93 @@
+"""The batched persistent form (one launch over many layouts, items and filters, fed by a
+device work queue) against the oracle, bit for bit; hipGraph replays checked one by one."""
+import numpy as np
+import pytest
+import torch
+
+import jagua_amd as J
+from jagua_amd import synth as S
+from oracle import oracle as O
+
+from helpers import engines, items
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batched_many_layouts(cuda):
+    scenes = [S.config1_scene(10 + i) for i in range(3)] + [S.config2_scene(seed=11, n_items=250)]
+    built = [engines(s) for s in scenes]
+    polys = [S.polygon(100 + i, 8, 200, 8.0, 30.0, 2) for i in range(5)] + [S.config1_queries(3)[0][0]]
+    prod, orc = items(polys)
+    sets, expect, first = [], [], 0
+    mats_all = []
+    filters = [None, "keys", "hashed"]
+    for li, (cde, keys, oc) in enumerate(built):
+        box = scenes[li].cde_bbox()
+        for ii in range(len(polys)):
+            n = 300 + 97 * ii + 13 * li
+            dts = S.transforms(1000 * li + ii, n, box, 0)
+            mats = J.compose(dts)
+            kind = filters[(li + ii) % 3]
+            if kind == "keys":
+                sub = keys[::4]
+                pf, of = J.HazKeyFilter(sub), O.make_filter(1, keys=[k.idx for k in sub])
+            elif kind == "hashed":
+                pf = J.HashedFilter(77 + li, 0.1)
+                of = O.make_filter(2, pf.seed, pf.threshold)
+            else:
+                pf, of = None, None
+            sets.append((cde, prod[ii], first, n, pf))
+            expect.append(O.query(oc, orc[ii], mats=mats, mode=1, filt=of, threads=8)["status"])
+            mats_all.append(mats)
+            first += n
+    plan = J.QueryPlan(sets)
+    m = torch.from_numpy(np.concatenate(mats_all)).to(cuda)
+    out = torch.full((first,), 255, dtype=torch.uint8, device=cuda)
+    plan.detect(m, out)
+    np.testing.assert_array_equal(out.cpu().numpy(), np.concatenate(expect))
+    # graph replays, with eager work and synchronisation between them
+    for r in range(3):
+        out.fill_(255)
+        torch.cuda.synchronize()
+        plan.detect_graph(m, out, replays=1)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(out.cpu().numpy(), np.concatenate(expect))
+    # the plan follows an engine update (re-flattened image)
+    cde, keys, oc = built[0]
+    cde.deregister_hazard_by_key(keys[0])
+    oc.deregister_key(oc.keys[0])
+    exp0 = O.query(oc, orc[0], mats=mats_all[0], mode=1, threads=8)["status"]
+    plan.detect(m, out)
+    np.testing.assert_array_equal(out[: len(exp0)].cpu().numpy(), exp0)
+
+
+@pytest.mark.parametrize("chunk", [1, 7, 64])
+def test_batched_chunks(cuda, chunk):
+    sc = S.config1_scene(20)
+    cde, keys, oc = engines(sc)
+    polys, dts = S.config1_queries(20, n=3000)
+    prod, orc = items(polys)
+    mats = J.compose(dts)
+    plan = J.QueryPlan([(cde, prod[0], 0, len(mats), None)], chunk=chunk, workgroups=7)
+    out = torch.empty(len(mats), dtype=torch.uint8, device=cuda)
+    plan.detect(torch.from_numpy(mats).to(cuda), out)
+    ref = O.query(oc, orc[0], mats=mats, mode=1)["status"]
+    np.testing.assert_array_equal(out.cpu().numpy(), ref)
+
+
+def test_batched_poly_only_and_surrogate_only(cuda):
+    sc = S.config1_scene(21)
+    cde, keys, oc = engines(sc)
+    polys, dts = S.config1_queries(21, n=2048)
+    prod, orc = items(polys)
+    mats = J.compose(dts)
+    m = torch.from_numpy(mats).to(cuda)
+    ref = O.query(oc, orc[0], mats=mats, mode=1)
+    for flags in (J.QUERY_POLY, J.QUERY_SURROGATE):
+        plan = J.QueryPlan([(cde, prod[0], 0, len(mats), None)], flags=flags)
+        out = torch.empty(len(mats), dtype=torch.uint8, device=cuda)
+        plan.detect(m, out)
+        plain = cde.detect_batch(prod[0], m, flags=flags).cpu().numpy()
+        np.testing.assert_array_equal(out.cpu().numpy(), plain)
+        if flags == J.QUERY_SURROGATE:
+            np.testing.assert_array_equal(plain, ref["surr"])
