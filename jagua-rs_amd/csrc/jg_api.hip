@@ -646,18 +646,27 @@ extern "C" jg_status jg_cde_dump(const jg_cde* c, uint32_t* nodes, uint64_t node
 }
 
 // ------------------------------------------------------------------------------ ABI: queries
-extern "C" jg_status jg_detect_batch(jg_cde* c, const jg_spolygon* item, const float* d_m6, uint64_t n,
-                                     const jg_hazard_filter* filter, uint32_t flags, uint8_t* d_status, void* stream) {
-    if (!c || !item) return fail(JG_ERR_INVALID_ARG, "null argument");
-    if (n == 0) return JG_OK;
-    if (!d_m6 || !d_status) return fail(JG_ERR_INVALID_ARG, "null device buffer");
+// SPSurrogate::ff_poles slices poles[0..n_ff_poles] (sp_surrogate.rs:55-57): the reference
+// panics when the item has fewer poles; here the call is rejected before any GPU work.
+static jg_status check_item(const jg_spolygon* item, uint32_t flags) {
     if (flags == 0) flags = JG_QUERY_SURROGATE | JG_QUERY_POLY;
     const Polygon& p = item->poly;
     if (flags & JG_QUERY_SURROGATE) {
         if (!p.sur.present) return fail(JG_ERR_SURROGATE, "surrogate not generated");
         if (p.sur.cfg.n_ff_poles > p.sur.poles.size()) return fail(JG_ERR_SURROGATE, "ff_poles slice out of range");
     }
-    jg_status s = cde_upload(c);
+    return JG_OK;
+}
+
+extern "C" jg_status jg_detect_batch(jg_cde* c, const jg_spolygon* item, const float* d_m6, uint64_t n,
+                                     const jg_hazard_filter* filter, uint32_t flags, uint8_t* d_status, void* stream) {
+    if (!c || !item) return fail(JG_ERR_INVALID_ARG, "null argument");
+    if (n == 0) return JG_OK;
+    if (!d_m6 || !d_status) return fail(JG_ERR_INVALID_ARG, "null device buffer");
+    if (flags == 0) flags = JG_QUERY_SURROGATE | JG_QUERY_POLY;
+    jg_status s = check_item(item, flags);
+    if (s != JG_OK) return s;
+    s = cde_upload(c);
     if (s != JG_OK) return s;
     s = item_upload(const_cast<jg_spolygon*>(item));
     if (s != JG_OK) return s;
@@ -681,10 +690,9 @@ extern "C" jg_status jg_collect_batch(jg_cde* c, const jg_spolygon* item, const 
     if (!c || !item) return fail(JG_ERR_INVALID_ARG, "null argument");
     if (n == 0) return JG_OK;
     if (!d_m6 || !d_count || !d_hash) return fail(JG_ERR_INVALID_ARG, "null device buffer");
-    const Polygon& p = item->poly;
-    if (!p.sur.present) return fail(JG_ERR_SURROGATE, "surrogate not generated");
-    if (p.sur.cfg.n_ff_poles > p.sur.poles.size()) return fail(JG_ERR_SURROGATE, "ff_poles slice out of range");
-    jg_status s = cde_upload(c);
+    jg_status s = check_item(item, JG_QUERY_SURROGATE | JG_QUERY_POLY);
+    if (s != JG_OK) return s;
+    s = cde_upload(c);
     if (s != JG_OK) return s;
     s = item_upload(const_cast<jg_spolygon*>(item));
     if (s != JG_OK) return s;
@@ -711,8 +719,11 @@ struct DevBuf {
 
 extern "C" jg_status jg_detect_batch_host(jg_cde* c, const jg_spolygon* item, const float* m6, uint64_t n,
                                           const jg_hazard_filter* filter, uint32_t flags, uint8_t* status) {
+    if (!c || !item || (n && (!m6 || !status))) return fail(JG_ERR_INVALID_ARG, "null argument");
     if (n == 0) return JG_OK;
-    jg_status s = device_ready();
+    jg_status s = check_item(item, flags);
+    if (s != JG_OK) return s;
+    s = device_ready();
     if (s != JG_OK) return s;
     DevBuf dm, ds;
     HIP_TRY(hipMalloc(&dm.p, sizeof(float) * 6 * n));
@@ -726,8 +737,11 @@ extern "C" jg_status jg_detect_batch_host(jg_cde* c, const jg_spolygon* item, co
 
 extern "C" jg_status jg_collect_batch_host(jg_cde* c, const jg_spolygon* item, const float* m6, uint64_t n,
                                            uint32_t capacity, uint32_t* count, uint64_t* hash, uint32_t* keys) {
+    if (!c || !item || (n && (!m6 || !count || !hash))) return fail(JG_ERR_INVALID_ARG, "null argument");
     if (n == 0) return JG_OK;
-    jg_status s = device_ready();
+    jg_status s = check_item(item, 0);
+    if (s != JG_OK) return s;
+    s = device_ready();
     if (s != JG_OK) return s;
     DevBuf dm, dc, dh, dk;
     HIP_TRY(hipMalloc(&dm.p, sizeof(float) * 6 * n));

@@ -1,0 +1,169 @@
+"""Host side of the product (polygon construction, quadtree register / deregister /
+restore, BFS flattening) against the oracle: identical trees node by node, identical
+polygon construction bit for bit, the reference's invariants (util/assertions.rs), and the
+error paths the reference takes (bail!, expect, panic) reported as error codes.  No GPU."""
+import numpy as np
+import pytest
+
+import jagua_amd as J
+from jagua_amd import synth as S
+from oracle import oracle as O
+
+from helpers import assert_same_item, assert_same_tree, engines
+
+
+@pytest.mark.parametrize("depth", [0, 3, 5, 10])
+def test_tree_matches_oracle(depth):
+    sc = S.config1_scene(30)
+    sc.depth = depth
+    cde, keys, oc = engines(sc)
+    assert_same_tree(cde, oc)
+    assert oc.invariants() == 3
+
+
+def test_tree_matches_oracle_holes_and_star_container():
+    sc = S.generate_scene(depth=7, seed=31, container_kind=1, cw=300.0, ch=500.0, container_nv_lo=64,
+                          container_nv_hi=300, n_holes=3, hole_kind=1, hole_lo=20.0, hole_hi=60.0, n_items=60,
+                          item_nv_lo=8, item_nv_hi=64, item_r_lo=8.0, item_r_hi=25.0, item_shape=2, min_sep=0.5)
+    assert len(sc.items) > 20
+    cde, keys, oc = engines(sc)
+    assert_same_tree(cde, oc)
+    assert oc.invariants() == 3
+
+
+def test_update_sequences_match_and_equal_fresh():
+    """lbf/tests/tests.rs:48-66: remove random items, save, add, restore; after every step the
+    product tree equals the oracle's, and the incrementally updated tree equals a fresh one
+    (util/assertions.rs:91-104)."""
+    sc = S.config1_scene(32)
+    sc.depth = 6
+    cde, keys, oc = engines(sc)
+    rng = np.random.default_rng(0)
+    snap, osnap = cde.save(), oc.save()
+    live = list(range(len(keys)))
+    for step in range(8):
+        i = live.pop(int(rng.integers(len(live))))
+        cde.deregister_hazard_by_key(keys[i])
+        oc.deregister_key(oc.keys[i])
+        assert_same_tree(cde, oc)
+    # add new items
+    extra = S.config1_scene(33).items[:5]
+    for j, (loc, dt) in enumerate(extra):
+        pk = 1000 + j
+        cde.register_hazard(J.Hazard(J.HazardEntity.placed_item(pk, dt, pk),
+                                     J.SPolygon.new(loc).transform_clone(dt), True))
+        oc.register_item(pk, loc, dt, pk)
+        assert_same_tree(cde, oc)
+    cde.restore(snap)
+    oc.restore(osnap)
+    assert_same_tree(cde, oc)
+    fresh = O.OracleCDE(sc)
+    assert O.lib().or_cde_trees_match(oc.h, fresh.h) == 1
+    assert oc.invariants() == 3
+
+
+def test_deregister_by_entity():
+    sc = S.config1_scene(34)
+    cde, keys, oc = engines(sc)
+    loc, dt = sc.items[7]
+    cde.deregister_hazard_by_entity(J.HazardEntity.placed_item(7, dt, 7))
+    oc.deregister_item(7, dt, 7)
+    assert_same_tree(cde, oc)
+    with pytest.raises(J.JaguaError):
+        cde.deregister_hazard_by_entity(J.HazardEntity.placed_item(7, dt, 7))
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_polygon_construction_matches_oracle(seed):
+    for shape in (0, 1, 2):
+        poly = S.polygon(seed * 13 + shape, 5, 200, 5.0, 50.0, shape)
+        item = J.SPolygon.new(poly).generate_surrogate()
+        assert_same_item(item, poly)
+
+
+def test_piers_match_oracle():
+    poly = S.polygon(99, 12, 12, 20.0, 40.0, 0)
+    item = J.SPolygon.new(poly).generate_surrogate(J.SPSurrogateConfig(n_ff_poles=2, n_ff_piers=2))
+    assert len(item.piers) == 2
+    assert_same_item(item, poly, n_ff_poles=2, n_ff_piers=2)
+
+
+def test_polygon_properties():
+    """SPolygon::new invariants pinned by closed forms: CCW normalisation, shoelace area,
+    brute-force diameter, POI inside with radius = distance to the boundary."""
+    poly = S.polygon(5, 30, 30, 10.0, 40.0, 0)
+    cw = poly[::-1].copy()
+    p = J.SPolygon.new(cw)
+    v = p.vertices.astype(np.float64)
+    area = 0.5 * np.sum(v[:, 0] * np.roll(v[:, 1], -1) - np.roll(v[:, 0], -1) * v[:, 1])
+    assert area > 0 and abs(area - p.area) < 1e-3 * area
+    d = np.sqrt(((v[:, None, :] - v[None, :, :]) ** 2).sum(-1)).max()
+    assert abs(d - p.diameter) < 1e-4 * d
+    cx, cy, r = p.poi.astype(np.float64)
+    e0, e1 = v, np.roll(v, -1, axis=0)
+    t = np.clip(((cx - e0[:, 0]) * (e1[:, 0] - e0[:, 0]) + (cy - e0[:, 1]) * (e1[:, 1] - e0[:, 1]))
+                / ((e1 - e0) ** 2).sum(1), 0, 1)
+    dist = np.sqrt((e0[:, 0] + t * (e1[:, 0] - e0[:, 0]) - cx) ** 2 + (e0[:, 1] + t * (e1[:, 1] - e0[:, 1]) - cy) ** 2)
+    assert abs(dist.min() - r) < 1e-3 * r
+    poles = p.generate_surrogate().poles
+    assert np.allclose(poles[0], p.poi)
+    assert (poles[1:, 2] <= poles[0, 2] + 1e-6).all()
+
+
+def test_error_paths():
+    with pytest.raises(J.JaguaError, match="-2"):
+        J.SPolygon.new([[0, 0], [1, 0], [0, 0], [0, 1]])            # duplicate points (simple_polygon.rs:45-47)
+    with pytest.raises(J.JaguaError, match="-2"):
+        J.SPolygon.new([[0, 0], [1, 1], [2, 2]])                    # no area (:49-50)
+    with pytest.raises(J.JaguaError, match="-2"):
+        J.SPolygon.new([[0, 0], [1, 1]])                            # fewer than 3 points (:42-44)
+    sq = J.SPolygon.new([[0, 0], [10, 0], [10, 10], [0, 10]])
+    with pytest.raises(J.JaguaError, match="-3"):                   # no Exterior (cd_engine.rs:44-48)
+        J.CDEngine([0, 0, 10, 10], [J.Hazard(J.HazardEntity.hole(0), sq, False)])
+    cde = J.CDEngine([0, 0, 10, 10], [J.Hazard(J.HazardEntity.exterior(), sq, False)])
+    with pytest.raises(J.JaguaError, match="-4"):                   # unknown key (cd_engine.rs:91)
+        cde.deregister_hazard_by_key(J.HazKey(7, 1))
+    with pytest.raises(J.JaguaError, match="-7"):
+        J.CDEngine([0, 0, 10, 10], [J.Hazard(J.HazardEntity.exterior(), sq, False)], J.CDEConfig(quadtree_depth=13))
+    # ff_poles slice beyond the poles (sp_surrogate.rs:56) is rejected before any GPU work
+    item = J.SPolygon.new([[0, 0], [4, 0], [4, 3], [0, 3]]).generate_surrogate(
+        J.SPSurrogateConfig(n_pole_limits=((2, 0.0), (2, 0.0), (2, 0.0)), n_ff_poles=5))
+    with pytest.raises(J.JaguaError, match="-5"):
+        cde.detect_batch(item, np.zeros((1, 6), np.float32))
+
+
+def test_queries_need_the_gpu():
+    """No CPU fallback: without a HIP device a query fails loudly."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    sq = J.SPolygon.new([[0, 0], [10, 0], [10, 10], [0, 10]])
+    cde = J.CDEngine([0, 0, 10, 10], [J.Hazard(J.HazardEntity.exterior(), sq, False)])
+    item = J.SPolygon.new([[0, 0], [1, 0], [1, 1], [0, 1]]).generate_surrogate()
+    with pytest.raises(J.JaguaError, match="no HIP device"):
+        cde.detect_batch(item, np.zeros((4, 6), np.float32))
+
+
+def test_flattened_tree_records():
+    """The BFS image: children of a node are 4 consecutive records one level down whose boxes
+    are the node's quadrants (rect.rs:141-151), and n_active_edges sums the partial edges."""
+    sc = S.config1_scene(35)
+    cde, keys, oc = engines(sc)
+    nodes, haz = cde.dump()
+    bb = nodes[:, 4:8].view(np.float32)
+    child = 1
+    off = 0
+    for i, n in enumerate(nodes):
+        k = int(n[2])
+        assert int(n[3]) == int(haz[off:off + k, 2].sum())
+        pres = haz[off:off + k, 1]
+        assert (np.diff(pres.astype(int)) <= 0).all()          # Entire (2) before Partial (1)
+        off += k
+        if n[1] == 4:
+            x0, y0, x1, y1 = bb[i]
+            mx, my = np.float32((x0 + x1) / np.float32(2)), np.float32((y0 + y1) / np.float32(2))
+            q = np.array([[mx, my, x1, y1], [x0, my, mx, y1], [x0, y0, mx, my], [mx, y0, x1, my]], np.float32)
+            assert np.array_equal(bb[child:child + 4], q)
+            assert (nodes[child:child + 4, 0] == n[0] - 1).all()
+            child += 4
+    assert child == len(nodes)
