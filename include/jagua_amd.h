@@ -236,22 +236,32 @@ jg_status jg_plan_detect_graph(jg_plan* plan, const float* d_mat6, uint8_t* d_st
 
 /* ---------------------------------------------------------------- overlap gradient */
 
-/* Pole-pair overlap quantity (north-star extension; no reference counterpart; its primitives
- * are Circle::separation_distance, circle.rs:184-200):
- *   Q = sum over candidate item pairs (a,b), poles i of a, j of b, of max(0, r_i + r_j - |c_i - c_j|)^2
- *     + sum over poles i, container edges e, of max(0, r_i - dist(c_i, e))^2
- * with c_i = R(theta_a) p_i + (x_a, y_a).  Forward writes Q (device f32[1]); backward writes
- * dQ/d(x, y, theta) per item (device f32[n_items][3]).  Pole transforms are recomputed in the
- * backward pass, not stored.
- * d_pose [n_items][3] (x, y, theta); d_poles [total][3] local (px, py, r);
- * d_pole_off [n_items+1]; d_pairs [n_pairs][2] item indices (a < b);
- * d_cedges [n_cedges][4] container edges (x1 y1 x2 y2). */
-jg_status jg_overlap_fwd(const float* d_pose, const float* d_poles, const uint32_t* d_pole_off, uint32_t n_items,
-                         const uint32_t* d_pairs, uint64_t n_pairs, const float* d_cedges, uint32_t n_cedges,
-                         float* d_q, void* stream);
-jg_status jg_overlap_bwd(const float* d_pose, const float* d_poles, const uint32_t* d_pole_off, uint32_t n_items,
-                         const uint32_t* d_pairs, uint64_t n_pairs, const float* d_cedges, uint32_t n_cedges,
-                         float* d_grad, void* stream);
+/* Pole-pair overlap quantity (north-star extension; no reference counterpart; its
+ * primitives are Circle::separation_distance for circles and for edges, circle.rs:184-229):
+ *   Q = sum over candidate item pairs (a,b), poles i of a, j of b, with |c_i - c_j| < r_i + r_j,
+ *         of (sqrt((r_i + r_j)^2) - |c_i - c_j|)^2
+ *     + sum over poles i and container edges e with dist(c_i, e) < r_i of (r_i - dist(c_i, e))^2
+ * where c_i = R(theta_a) p_i + (x_a, y_a) for item a's pose (x, y, theta) and local pole
+ * (p_i, r_i).  Forward and backward are FP32 kernels; the backward recomputes the pole
+ * transforms instead of storing them.  Partial sums are reduced in a fixed order
+ * (deterministic run to run). */
+typedef struct jg_overlap jg_overlap;
+/* poles [total][3] (px, py, r) and pole_off [n_items + 1] (host); pairs [n_pairs][2] item
+ * indices (host); cedges [n_cedges][4] container edges x1 y1 x2 y2 (host). */
+jg_status jg_overlap_create(const float* poles, const uint32_t* pole_off, uint32_t n_items, const uint32_t* pairs,
+                            uint64_t n_pairs, const float* cedges, uint32_t n_cedges, jg_overlap** out);
+/* Replaces the candidate pairs (host [n_pairs][2]). */
+jg_status jg_overlap_set_pairs(jg_overlap* ov, const uint32_t* pairs, uint64_t n_pairs);
+/* Forward: d_pose device [n_items][3] (x, y, theta); d_q device f32[1]. */
+jg_status jg_overlap_fwd(jg_overlap* ov, const float* d_pose, float* d_q, void* stream);
+/* Backward: dQ/d(x, y, theta) per item into d_grad device [n_items][3]. */
+jg_status jg_overlap_bwd(jg_overlap* ov, const float* d_pose, float* d_grad, void* stream);
+void jg_overlap_free(jg_overlap* ov);
+/* Candidate pairs for the overlap quantity: items whose bounding circles (centre (x, y),
+ * radius rad) overlap, found with a uniform grid over `bbox` of 2^depth cells per side (the
+ * leaf cells of a quadtree of that depth).  Writes up to `cap` pairs (a < b) and their count. */
+jg_status jg_overlap_candidate_pairs(const float* pose, const float* rad, uint32_t n_items, const float bbox[4],
+                                     uint32_t depth, uint32_t* pairs_out, uint64_t cap, uint64_t* n_out);
 
 /* Version / build info. */
 const char* jg_version(void);

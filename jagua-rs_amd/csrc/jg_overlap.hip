@@ -1,16 +1,485 @@
-// Pole-pair overlap quantity: forward and backward kernels (jg_overlap_*).
+// Pole-pair overlap quantity and its gradient with respect to every item's (x, y, theta).
+//
+// Forward:  one wavefront per candidate item pair (lanes = poles of item a, item b's
+//           transformed poles staged in LDS) and one per item for the container edges; each
+//           wave writes one partial; a two-pass fixed-order reduction (f64 accumulation)
+//           produces Q.  Penetrations follow Circle::separation_distance (circle.rs:184-200:
+//           sqrt((r1+r2)^2) - sqrt(d^2)) and the circle–edge distance (circle.rs:216-229).
+// Backward: the same waves recompute the pole transforms, accumulate dQ/dc per pole and
+//           chain it to (x, y, theta); per-pair and per-item partial gradients are gathered
+//           per item through a CSR adjacency in a fixed order.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
 
 #include "../../include/jagua_amd.h"
 #include "jg_internal.hpp"
 
 using namespace jg;
 
-extern "C" jg_status jg_overlap_fwd(const float*, const float*, const uint32_t*, uint32_t, const uint32_t*, uint64_t,
-                                    const float*, uint32_t, float*, void*) {
-    return fail(JG_ERR_INTERNAL, "overlap kernels are not built yet");
+namespace {
+
+constexpr int OV_WAVES = 4;   // waves per workgroup
+
+__device__ __forceinline__ float wsum(float v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
-extern "C" jg_status jg_overlap_bwd(const float*, const float*, const uint32_t*, uint32_t, const uint32_t*, uint64_t,
-                                    const float*, uint32_t, float*, void*) {
-    return fail(JG_ERR_INTERNAL, "overlap kernels are not built yet");
+
+// world pole: c = R(theta) p + t; also returns the rotated local offset rp = R(theta) p
+__device__ __forceinline__ void pole_world(float3 pose, float s, float c, float4 p, float2& cw, float2& rp) {
+    rp = make_float2(c * p.x - s * p.y, s * p.x + c * p.y);
+    cw = make_float2(rp.x + pose.x, rp.y + pose.y);
+}
+
+// closest point on segment e to point (px, py) (edge.rs:63-87)
+__device__ __forceinline__ float2 closest_on_edge(float4 e, float px, float py) {
+    const float a = px - e.x, b = py - e.y, cx = e.z - e.x, cy = e.w - e.y;
+    const float dot = a * cx + b * cy;
+    const float len_sq = cx * cx + cy * cy;
+    float t = -1.0f;
+    if (len_sq != 0.0f) t = dot / len_sq;
+    if (t < 0.0f) return make_float2(e.x, e.y);
+    if (t > 1.0f) return make_float2(e.z, e.w);
+    return make_float2(e.x + t * cx, e.y + t * cy);
+}
+
+struct OvView {
+    const float4* poles;       // px, py, r, 0
+    const uint32_t* off;       // n_items + 1
+    const uint2* pairs;
+    const float4* cedges;
+    uint32_t n_items, n_cedges;
+    uint64_t n_pairs;
+};
+
+}  // namespace
+
+// ---- forward -------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_ov_fwd_pairs(OvView v, const float* __restrict__ pose3,
+                                                      float* __restrict__ partial) {
+    __shared__ float4 lds[OV_WAVES][64];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint64_t pid = (uint64_t)blockIdx.x * OV_WAVES + wave;
+    if (pid >= v.n_pairs) return;   // whole waves exit together; no block barrier below
+    const uint2 pr = v.pairs[pid];
+    const float3 pa = make_float3(pose3[3 * pr.x], pose3[3 * pr.x + 1], pose3[3 * pr.x + 2]);
+    const float3 pb = make_float3(pose3[3 * pr.y], pose3[3 * pr.y + 1], pose3[3 * pr.y + 2]);
+    float sa, ca, sb, cb;
+    sincosf(pa.z, &sa, &ca);
+    sincosf(pb.z, &sb, &cb);
+    const uint32_t a0 = v.off[pr.x], na = v.off[pr.x + 1] - a0;
+    const uint32_t b0 = v.off[pr.y], nb = v.off[pr.y + 1] - b0;
+    float acc = 0.0f;
+    for (uint32_t jb = 0; jb < nb; jb += 64) {
+        const uint32_t jn = min(64u, nb - jb);
+        if (lane < jn) {
+            float2 cw, rp;
+            const float4 p = v.poles[b0 + jb + lane];
+            pole_world(pb, sb, cb, p, cw, rp);
+            lds[wave][lane] = make_float4(cw.x, cw.y, p.z, 0.0f);
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (uint32_t i = lane; i < na; i += 64) {
+            float2 ci, rpi;
+            const float4 p = v.poles[a0 + i];
+            pole_world(pa, sa, ca, p, ci, rpi);
+            for (uint32_t j = 0; j < jn; j++) {
+                const float4 q = lds[wave][j];
+                const float dx = ci.x - q.x, dy = ci.y - q.y;
+                const float sq = dx * dx + dy * dy;
+                const float rs = p.z + q.z;
+                const float sqr = rs * rs;
+                if (sq < sqr) {
+                    const float pen = sqrtf(sqr) - sqrtf(sq);
+                    acc += pen * pen;
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    acc = wsum(acc);
+    if (lane == 0) partial[pid] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_ov_fwd_container(OvView v, const float* __restrict__ pose3,
+                                                          float* __restrict__ partial) {
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t item = blockIdx.x * OV_WAVES + wave;
+    if (item >= v.n_items) return;
+    const float3 pa = make_float3(pose3[3 * item], pose3[3 * item + 1], pose3[3 * item + 2]);
+    float s, c;
+    sincosf(pa.z, &s, &c);
+    const uint32_t a0 = v.off[item], na = v.off[item + 1] - a0;
+    float acc = 0.0f;
+    for (uint32_t i = lane; i < na; i += 64) {
+        float2 ci, rp;
+        const float4 p = v.poles[a0 + i];
+        pole_world(pa, s, c, p, ci, rp);
+        for (uint32_t e = 0; e < v.n_cedges; e++) {
+            const float4 ed = v.cedges[e];
+            const float2 q = closest_on_edge(ed, ci.x, ci.y);
+            const float dx = ci.x - q.x, dy = ci.y - q.y;
+            const float d = sqrtf(dx * dx + dy * dy);
+            if (d < p.z) {
+                const float pen = p.z - d;
+                acc += pen * pen;
+            }
+        }
+    }
+    acc = wsum(acc);
+    if (lane == 0) partial[item] = acc;
+}
+
+// fixed-order two-pass reduction: each block sums a contiguous slice in f64
+__global__ __launch_bounds__(256) void k_ov_reduce1(const float* __restrict__ x, uint64_t n, uint64_t per_block,
+                                                    double* __restrict__ out) {
+    __shared__ double sh[256];
+    const uint64_t lo = (uint64_t)blockIdx.x * per_block;
+    const uint64_t hi = min(n, lo + per_block);
+    double acc = 0.0;
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += 256) acc += (double)x[i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void k_ov_reduce2(const double* __restrict__ x, uint32_t n, float* __restrict__ q) {
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += 256) acc += x[i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) q[0] = (float)sh[0];
+}
+
+// ---- backward ------------------------------------------------------------------------
+// out[pid] = (dQ/dx_a, dQ/dy_a, dQ/dtheta_a, dQ/dx_b, dQ/dy_b, dQ/dtheta_b)
+__global__ __launch_bounds__(256) void k_ov_bwd_pairs(OvView v, const float* __restrict__ pose3,
+                                                      float* __restrict__ out6) {
+    __shared__ float4 lds_c[OV_WAVES][64];
+    __shared__ float2 lds_r[OV_WAVES][64];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint64_t pid = (uint64_t)blockIdx.x * OV_WAVES + wave;
+    if (pid >= v.n_pairs) return;
+    const uint2 pr = v.pairs[pid];
+    const float3 pa = make_float3(pose3[3 * pr.x], pose3[3 * pr.x + 1], pose3[3 * pr.x + 2]);
+    const float3 pb = make_float3(pose3[3 * pr.y], pose3[3 * pr.y + 1], pose3[3 * pr.y + 2]);
+    float sa, ca, sb, cb;
+    sincosf(pa.z, &sa, &ca);
+    sincosf(pb.z, &sb, &cb);
+    const uint32_t a0 = v.off[pr.x], na = v.off[pr.x + 1] - a0;
+    const uint32_t b0 = v.off[pr.y], nb = v.off[pr.y + 1] - b0;
+    float gxa = 0, gya = 0, gta = 0, gxb = 0, gyb = 0, gtb = 0;
+    for (uint32_t jb = 0; jb < nb; jb += 64) {
+        const uint32_t jn = min(64u, nb - jb);
+        if (lane < jn) {
+            float2 cw, rp;
+            const float4 p = v.poles[b0 + jb + lane];
+            pole_world(pb, sb, cb, p, cw, rp);
+            lds_c[wave][lane] = make_float4(cw.x, cw.y, p.z, 0.0f);
+            lds_r[wave][lane] = rp;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (uint32_t i = lane; i < na; i += 64) {
+            float2 ci, rpi;
+            const float4 p = v.poles[a0 + i];
+            pole_world(pa, sa, ca, p, ci, rpi);
+            float gcx = 0.0f, gcy = 0.0f;
+            for (uint32_t j = 0; j < jn; j++) {
+                const float4 q = lds_c[wave][j];
+                const float dx = ci.x - q.x, dy = ci.y - q.y;
+                const float sq = dx * dx + dy * dy;
+                const float rs = p.z + q.z;
+                const float sqr = rs * rs;
+                if (sq < sqr && sq > 0.0f) {
+                    const float d = sqrtf(sq);
+                    const float pen = sqrtf(sqr) - d;
+                    const float k = -2.0f * pen / d;        // dQ/dc_i = k (c_i - c_j)
+                    const float gx = k * dx, gy = k * dy;
+                    gcx += gx;
+                    gcy += gy;
+                    const float2 rpj = lds_r[wave][j];      // dc_j/dtheta_b = (-rp_j.y, rp_j.x)
+                    gxb -= gx;
+                    gyb -= gy;
+                    gtb += gx * rpj.y - gy * rpj.x;
+                }
+            }
+            gxa += gcx;
+            gya += gcy;
+            gta += -gcx * rpi.y + gcy * rpi.x;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    gxa = wsum(gxa); gya = wsum(gya); gta = wsum(gta);
+    gxb = wsum(gxb); gyb = wsum(gyb); gtb = wsum(gtb);
+    if (lane == 0) {
+        float* o = out6 + 6 * pid;
+        o[0] = gxa; o[1] = gya; o[2] = gta; o[3] = gxb; o[4] = gyb; o[5] = gtb;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ov_bwd_container(OvView v, const float* __restrict__ pose3,
+                                                          float* __restrict__ out3) {
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t item = blockIdx.x * OV_WAVES + wave;
+    if (item >= v.n_items) return;
+    const float3 pa = make_float3(pose3[3 * item], pose3[3 * item + 1], pose3[3 * item + 2]);
+    float s, c;
+    sincosf(pa.z, &s, &c);
+    const uint32_t a0 = v.off[item], na = v.off[item + 1] - a0;
+    float gx = 0, gy = 0, gt = 0;
+    for (uint32_t i = lane; i < na; i += 64) {
+        float2 ci, rp;
+        const float4 p = v.poles[a0 + i];
+        pole_world(pa, s, c, p, ci, rp);
+        float gcx = 0.0f, gcy = 0.0f;
+        for (uint32_t e = 0; e < v.n_cedges; e++) {
+            const float2 q = closest_on_edge(v.cedges[e], ci.x, ci.y);
+            const float dx = ci.x - q.x, dy = ci.y - q.y;
+            const float sq = dx * dx + dy * dy;
+            const float d = sqrtf(sq);
+            if (d < p.z && d > 0.0f) {
+                const float k = -2.0f * (p.z - d) / d;
+                gcx += k * dx;
+                gcy += k * dy;
+            }
+        }
+        gx += gcx;
+        gy += gcy;
+        gt += -gcx * rp.y + gcy * rp.x;
+    }
+    gx = wsum(gx); gy = wsum(gy); gt = wsum(gt);
+    if (lane == 0) {
+        out3[3 * item] = gx; out3[3 * item + 1] = gy; out3[3 * item + 2] = gt;
+    }
+}
+
+// per item, in a fixed order: container term, then its pairs in CSR order (f64 accumulation)
+__global__ __launch_bounds__(256) void k_ov_bwd_gather(uint32_t n_items, const uint32_t* __restrict__ adj_off,
+                                                       const uint32_t* __restrict__ adj, const float* __restrict__ pg6,
+                                                       const float* __restrict__ cg3, float* __restrict__ grad) {
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= n_items) return;
+    double gx = cg3[3 * item], gy = cg3[3 * item + 1], gt = cg3[3 * item + 2];
+    for (uint32_t k = adj_off[item]; k < adj_off[item + 1]; k++) {
+        const uint32_t code = adj[k];               // pair << 1 | role (0 = a, 1 = b)
+        const float* g = pg6 + 6 * (uint64_t)(code >> 1) + 3 * (code & 1u);
+        gx += g[0]; gy += g[1]; gt += g[2];
+    }
+    grad[3 * item] = (float)gx;
+    grad[3 * item + 1] = (float)gy;
+    grad[3 * item + 2] = (float)gt;
+}
+
+// ---------------------------------------------------------------------------------------
+struct jg_overlap {
+    uint32_t n_items = 0, n_cedges = 0;
+    uint64_t n_pairs = 0;
+    DevBlob geo;                  // poles, offsets, container edges
+    DevBlob pairs;                // pairs + CSR adjacency
+    DevBlob scratch;              // partials, per-pair gradients, block sums
+    OvView view{};
+    const uint32_t* adj_off = nullptr;
+    const uint32_t* adj = nullptr;
+};
+
+namespace {
+
+size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
+
+jg_status ov_upload_pairs(jg_overlap* ov, const uint32_t* pairs, uint64_t n_pairs) {
+    for (uint64_t i = 0; i < n_pairs; i++)
+        if (pairs[2 * i] >= ov->n_items || pairs[2 * i + 1] >= ov->n_items || pairs[2 * i] == pairs[2 * i + 1])
+            return fail(JG_ERR_INVALID_ARG, "pair index out of range or self pair");
+    if (n_pairs >= (1ull << 31)) return fail(JG_ERR_INVALID_ARG, "too many pairs");
+    // CSR adjacency: per item its (pair, role) codes in pair order
+    std::vector<uint32_t> off(ov->n_items + 1, 0), adj(2 * n_pairs);
+    for (uint64_t i = 0; i < n_pairs; i++) { off[pairs[2 * i] + 1]++; off[pairs[2 * i + 1] + 1]++; }
+    for (uint32_t i = 0; i < ov->n_items; i++) off[i + 1] += off[i];
+    std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+    for (uint64_t i = 0; i < n_pairs; i++) {
+        adj[fill[pairs[2 * i]]++] = (uint32_t)(i << 1);
+        adj[fill[pairs[2 * i + 1]]++] = (uint32_t)(i << 1) | 1u;
+    }
+    const size_t o_pairs = 0, o_off = al(8 * n_pairs + 8), o_adj = o_off + al(4 * off.size());
+    const size_t total = o_adj + al(4 * adj.size() + 4);
+    std::vector<uint8_t> host(total, 0);
+    std::memcpy(host.data() + o_pairs, pairs, 8 * n_pairs);
+    std::memcpy(host.data() + o_off, off.data(), 4 * off.size());
+    std::memcpy(host.data() + o_adj, adj.data(), 4 * adj.size());
+    jg_status s = ov->pairs.reserve(total);
+    if (s != JG_OK) return s;
+    s = hip_check(hipMemcpy(ov->pairs.ptr, host.data(), total, hipMemcpyHostToDevice), "hipMemcpy pairs");
+    if (s != JG_OK) return s;
+    uint8_t* b = (uint8_t*)ov->pairs.ptr;
+    ov->view.pairs = (const uint2*)(b + o_pairs);
+    ov->adj_off = (const uint32_t*)(b + o_off);
+    ov->adj = (const uint32_t*)(b + o_adj);
+    ov->n_pairs = n_pairs;
+    ov->view.n_pairs = n_pairs;
+    const size_t need = al(4 * (n_pairs + ov->n_items) + 4) + al(24 * n_pairs + 24) + al(12 * ov->n_items + 12) +
+                        al(8 * 4096);
+    return ov->scratch.reserve(need);
+}
+
+struct Scratch {
+    float* partial;    // n_pairs + n_items
+    float* pg6;        // n_pairs * 6
+    float* cg3;        // n_items * 3
+    double* blocks;    // up to 4096
+};
+Scratch scratch_of(jg_overlap* ov) {
+    uint8_t* b = (uint8_t*)ov->scratch.ptr;
+    Scratch s;
+    size_t o = 0;
+    s.partial = (float*)(b + o); o += al(4 * (ov->n_pairs + ov->n_items) + 4);
+    s.pg6 = (float*)(b + o); o += al(24 * ov->n_pairs + 24);
+    s.cg3 = (float*)(b + o); o += al(12 * ov->n_items + 12);
+    s.blocks = (double*)(b + o);
+    return s;
+}
+
+uint32_t blocks_for(uint64_t waves) { return (uint32_t)((waves + OV_WAVES - 1) / OV_WAVES); }
+
+}  // namespace
+
+extern "C" jg_status jg_overlap_create(const float* poles, const uint32_t* pole_off, uint32_t n_items,
+                                       const uint32_t* pairs, uint64_t n_pairs, const float* cedges, uint32_t n_cedges,
+                                       jg_overlap** out) {
+    if (!out || !pole_off || (n_items && !poles) || (n_pairs && !pairs) || (n_cedges && !cedges))
+        return fail(JG_ERR_INVALID_ARG, "null argument");
+    jg_status s = device_ready();
+    if (s != JG_OK) return s;
+    for (uint32_t i = 0; i < n_items; i++)
+        if (pole_off[i + 1] < pole_off[i]) return fail(JG_ERR_INVALID_ARG, "pole offsets must be non-decreasing");
+    auto ov = std::make_unique<jg_overlap>();
+    ov->n_items = n_items;
+    ov->n_cedges = n_cedges;
+    const uint32_t total = pole_off[n_items];
+    const size_t o_p = 0, o_off = al(16 * (size_t)total + 16), o_e = o_off + al(4 * ((size_t)n_items + 1));
+    const size_t bytes = o_e + al(16 * (size_t)n_cedges + 16);
+    std::vector<uint8_t> host(bytes, 0);
+    float4* p4 = (float4*)(host.data() + o_p);
+    for (uint32_t i = 0; i < total; i++) p4[i] = make_float4(poles[3 * i], poles[3 * i + 1], poles[3 * i + 2], 0.0f);
+    std::memcpy(host.data() + o_off, pole_off, 4 * ((size_t)n_items + 1));
+    std::memcpy(host.data() + o_e, cedges, 16 * (size_t)n_cedges);
+    s = ov->geo.reserve(bytes);
+    if (s != JG_OK) return s;
+    s = hip_check(hipMemcpy(ov->geo.ptr, host.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy overlap");
+    if (s != JG_OK) return s;
+    uint8_t* b = (uint8_t*)ov->geo.ptr;
+    ov->view.poles = (const float4*)(b + o_p);
+    ov->view.off = (const uint32_t*)(b + o_off);
+    ov->view.cedges = (const float4*)(b + o_e);
+    ov->view.n_items = n_items;
+    ov->view.n_cedges = n_cedges;
+    s = ov_upload_pairs(ov.get(), pairs, n_pairs);
+    if (s != JG_OK) return s;
+    *out = ov.release();
+    return JG_OK;
+}
+
+extern "C" jg_status jg_overlap_set_pairs(jg_overlap* ov, const uint32_t* pairs, uint64_t n_pairs) {
+    if (!ov || (n_pairs && !pairs)) return fail(JG_ERR_INVALID_ARG, "null argument");
+    jg_status s = hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    if (s != JG_OK) return s;
+    return ov_upload_pairs(ov, pairs, n_pairs);
+}
+
+extern "C" void jg_overlap_free(jg_overlap* ov) { delete ov; }
+
+extern "C" jg_status jg_overlap_fwd(jg_overlap* ov, const float* d_pose, float* d_q, void* stream) {
+    if (!ov || !d_pose || !d_q) return fail(JG_ERR_INVALID_ARG, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    Scratch s = scratch_of(ov);
+    if (ov->n_pairs)
+        hipLaunchKernelGGL(k_ov_fwd_pairs, dim3(blocks_for(ov->n_pairs)), dim3(64 * OV_WAVES), 0, st, ov->view, d_pose,
+                           s.partial);
+    if (ov->n_items)
+        hipLaunchKernelGGL(k_ov_fwd_container, dim3(blocks_for(ov->n_items)), dim3(64 * OV_WAVES), 0, st, ov->view,
+                           d_pose, s.partial + ov->n_pairs);
+    const uint64_t n = ov->n_pairs + ov->n_items;
+    const uint64_t per = std::max<uint64_t>(1024, (n + 4095) / 4096);
+    const uint32_t nb = (uint32_t)std::max<uint64_t>(1, (n + per - 1) / per);
+    hipLaunchKernelGGL(k_ov_reduce1, dim3(nb), dim3(256), 0, st, s.partial, n, per, s.blocks);
+    hipLaunchKernelGGL(k_ov_reduce2, dim3(1), dim3(256), 0, st, s.blocks, nb, d_q);
+    return hip_check(hipGetLastError(), "overlap forward launch");
+}
+
+extern "C" jg_status jg_overlap_bwd(jg_overlap* ov, const float* d_pose, float* d_grad, void* stream) {
+    if (!ov || !d_pose || !d_grad) return fail(JG_ERR_INVALID_ARG, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    Scratch s = scratch_of(ov);
+    if (ov->n_pairs)
+        hipLaunchKernelGGL(k_ov_bwd_pairs, dim3(blocks_for(ov->n_pairs)), dim3(64 * OV_WAVES), 0, st, ov->view, d_pose,
+                           s.pg6);
+    if (ov->n_items) {
+        hipLaunchKernelGGL(k_ov_bwd_container, dim3(blocks_for(ov->n_items)), dim3(64 * OV_WAVES), 0, st, ov->view,
+                           d_pose, s.cg3);
+        hipLaunchKernelGGL(k_ov_bwd_gather, dim3((ov->n_items + 255) / 256), dim3(256), 0, st, ov->n_items, ov->adj_off,
+                           ov->adj, s.pg6, s.cg3, d_grad);
+    }
+    return hip_check(hipGetLastError(), "overlap backward launch");
+}
+
+// uniform grid broad phase on bounding circles (host)
+extern "C" jg_status jg_overlap_candidate_pairs(const float* pose, const float* rad, uint32_t n_items,
+                                                const float bbox[4], uint32_t depth, uint32_t* pairs_out, uint64_t cap,
+                                                uint64_t* n_out) {
+    if (!pose || !rad || !bbox || !n_out) return fail(JG_ERR_INVALID_ARG, "null argument");
+    if (depth > 12) return fail(JG_ERR_DEPTH, "grid depth above 12");
+    const uint32_t side = 1u << depth;
+    const float w = (bbox[2] - bbox[0]) / (float)side, h = (bbox[3] - bbox[1]) / (float)side;
+    std::vector<std::vector<uint32_t>> cells((size_t)side * side);
+    auto cell_range = [&](uint32_t i, int& x0, int& y0, int& x1, int& y1) {
+        const float x = pose[3 * i], y = pose[3 * i + 1], r = rad[i];
+        auto cl = [&](float v, float lo, float step) {
+            int c = (int)std::floor((v - lo) / step);
+            return std::max(0, std::min((int)side - 1, c));
+        };
+        x0 = cl(x - r, bbox[0], w); x1 = cl(x + r, bbox[0], w);
+        y0 = cl(y - r, bbox[1], h); y1 = cl(y + r, bbox[1], h);
+    };
+    for (uint32_t i = 0; i < n_items; i++) {
+        int x0, y0, x1, y1;
+        cell_range(i, x0, y0, x1, y1);
+        for (int yy = y0; yy <= y1; yy++)
+            for (int xx = x0; xx <= x1; xx++) cells[(size_t)yy * side + xx].push_back(i);
+    }
+    uint64_t n = 0;
+    std::vector<uint32_t> seen(n_items, 0xffffffffu);
+    for (uint32_t i = 0; i < n_items; i++) {
+        int x0, y0, x1, y1;
+        cell_range(i, x0, y0, x1, y1);
+        for (int yy = y0; yy <= y1; yy++)
+            for (int xx = x0; xx <= x1; xx++)
+                for (uint32_t j : cells[(size_t)yy * side + xx]) {
+                    if (j <= i || seen[j] == i) continue;
+                    seen[j] = i;
+                    const float dx = pose[3 * i] - pose[3 * j], dy = pose[3 * i + 1] - pose[3 * j + 1];
+                    const float rr = rad[i] + rad[j];
+                    if (dx * dx + dy * dy > rr * rr) continue;
+                    if (pairs_out && n < cap) { pairs_out[2 * n] = i; pairs_out[2 * n + 1] = j; }
+                    n++;
+                }
+    }
+    *n_out = n;
+    return n > cap && pairs_out ? fail(JG_ERR_CAPACITY, "pair buffer too small") : JG_OK;
 }

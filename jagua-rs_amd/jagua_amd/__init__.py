@@ -452,5 +452,63 @@ class QueryPlan:
         return out
 
 
+class PoleOverlap:
+    """The pole-pair overlap quantity Q (include/jagua_amd.h, jg_overlap_*) as a forward /
+    backward op pair over per-item poses (x, y, theta), on the GPU.
+
+    poles: [total, 3] local (px, py, r); pole_off: [n_items + 1]; pairs: [n_pairs, 2] item
+    indices; cedges: [n_edges, 4] container edges (x1, y1, x2, y2)."""
+
+    def __init__(self, poles, pole_off, pairs, cedges):
+        poles = _f32(poles).reshape(-1, 3)
+        off = np.ascontiguousarray(pole_off, dtype=np.uint32)
+        pr = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+        ce = _f32(cedges).reshape(-1, 4)
+        self.n_items = len(off) - 1
+        self._h = C.c_void_p()
+        check(N.lib().jg_overlap_create(_ptr(poles), _ptr(off, C.c_uint32), self.n_items, _ptr(pr, C.c_uint32), len(pr),
+                                        _ptr(ce), len(ce), C.byref(self._h)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and N._lib is not None:
+            N._lib.jg_overlap_free(self._h)
+            self._h = C.c_void_p()
+
+    def set_pairs(self, pairs):
+        pr = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+        check(N.lib().jg_overlap_set_pairs(self._h, _ptr(pr, C.c_uint32), len(pr)))
+
+    def forward(self, pose, out=None, stream=None):
+        """Q for pose (torch GPU f32 [n_items, 3]); returns a GPU f32 tensor of shape [1]."""
+        import torch
+        if out is None:
+            out = torch.empty(1, dtype=torch.float32, device=pose.device)
+        check(N.lib().jg_overlap_fwd(self._h, C.c_void_p(pose.data_ptr()), C.c_void_p(out.data_ptr()),
+                                     _stream_ptr(stream)))
+        return out
+
+    def backward(self, pose, out=None, stream=None):
+        """dQ/d(x, y, theta) per item (GPU f32 [n_items, 3])."""
+        import torch
+        if out is None:
+            out = torch.empty((self.n_items, 3), dtype=torch.float32, device=pose.device)
+        check(N.lib().jg_overlap_bwd(self._h, C.c_void_p(pose.data_ptr()), C.c_void_p(out.data_ptr()),
+                                     _stream_ptr(stream)))
+        return out
+
+
+def candidate_pairs(pose, radius, bbox, depth=8):
+    """Items whose bounding circles overlap (uniform grid of 2^depth cells per side)."""
+    pose = _f32(pose).reshape(-1, 3)
+    rad = _f32(radius).reshape(-1)
+    bb = _f32(bbox).reshape(4)
+    n = C.c_uint64()
+    check(N.lib().jg_overlap_candidate_pairs(_ptr(pose), _ptr(rad), len(pose), _ptr(bb), depth, None, 0, C.byref(n)))
+    out = np.empty((max(1, n.value), 2), dtype=np.uint32)
+    check(N.lib().jg_overlap_candidate_pairs(_ptr(pose), _ptr(rad), len(pose), _ptr(bb), depth,
+                                             _ptr(out, C.c_uint32), n.value, C.byref(n)))
+    return out[: n.value]
+
+
 def version():
     return N.lib().jg_version().decode()

@@ -93,8 +93,14 @@ _SIGS = {
     "jg_plan_free": (None, [c_vp]),
     "jg_plan_detect": (C.c_int32, [c_vp, c_vp, c_vp, c_vp]),
     "jg_plan_detect_graph": (C.c_int32, [c_vp, c_vp, c_vp, C.c_uint32, c_vp]),
-    "jg_overlap_fwd": (C.c_int32, [c_vp, c_vp, c_vp, C.c_uint32, c_vp, C.c_uint64, c_vp, C.c_uint32, c_vp, c_vp]),
-    "jg_overlap_bwd": (C.c_int32, [c_vp, c_vp, c_vp, C.c_uint32, c_vp, C.c_uint64, c_vp, C.c_uint32, c_vp, c_vp]),
+    "jg_overlap_create": (C.c_int32, [c_f32p, c_u32p, C.c_uint32, c_u32p, C.c_uint64, c_f32p, C.c_uint32,
+                                      C.POINTER(c_vp)]),
+    "jg_overlap_set_pairs": (C.c_int32, [c_vp, c_u32p, C.c_uint64]),
+    "jg_overlap_fwd": (C.c_int32, [c_vp, c_vp, c_vp, c_vp]),
+    "jg_overlap_bwd": (C.c_int32, [c_vp, c_vp, c_vp, c_vp]),
+    "jg_overlap_free": (None, [c_vp]),
+    "jg_overlap_candidate_pairs": (C.c_int32, [c_f32p, c_f32p, C.c_uint32, c_f32p, C.c_uint32, c_u32p, C.c_uint64,
+                                               c_u64p]),
     "jg_version": (C.c_char_p, []),
     "jg_last_error": (C.c_char_p, []),
 }

@@ -153,3 +153,41 @@ def config2_queries(seed=2, n_items=64, n_per_item=16384):
     polys = [polygon(seed * 100003 + i, 8, 512, 10.0, 30.0, 2) for i in range(n_items)]
     dts = [transforms(seed * 100003 + 7 * i + 1, n_per_item, (0.0, 0.0, 4000.0, 1500.0), 0) for i in range(n_items)]
     return polys, dts
+
+
+def config4_workload(seed=4, n_items=4096, width=2000.0, height=800.0, r_lo=6.0, r_hi=11.5, jitter=3.0,
+                     poles_lo=16, poles_hi=64, depth=8):
+    """Differentiable overlap: n_items star polygons (12-24 vertices, radius U[r_lo, r_hi]) each
+    carrying 16-64 surrogate poles (SPSurrogateConfig with n_pole_limits (n, 0.0): exactly n
+    inscribed circles), on a jittered grid over a width x height rectangle.  Returns a dict
+    with local poles [total, 3], pole_off, pose [n, 3], bounding radius, candidate pairs (grid
+    broad phase at quadtree depth `depth`) and the container's 4 edges."""
+    from . import candidate_pairs
+    rng = np.random.default_rng(seed)
+    nx = int(np.ceil(np.sqrt(n_items * width / height)))
+    ny = int(np.ceil(n_items / nx))
+    gx, gy = width / nx, height / ny
+    polys, n_poles = [], []
+    for i in range(n_items):
+        polys.append(polygon(seed * 1_000_003 + i, 12, 24, r_lo, r_hi, 0))
+        n_poles.append(int(rng.integers(poles_lo, poles_hi + 1)))
+
+    def mk(a):
+        p, n = a
+        sp = SPolygon.new(p).generate_surrogate(SPSurrogateConfig(((n, 0.0), (n, 0.0), (n, 0.0)), n_ff_poles=2))
+        return sp.poles
+
+    with cf.ThreadPoolExecutor(_threads()) as ex:
+        poles = list(ex.map(mk, zip(polys, n_poles)))
+    off = np.zeros(n_items + 1, dtype=np.uint32)
+    off[1:] = np.cumsum([len(p) for p in poles])
+    allp = np.concatenate(poles).astype(F32)
+    pose = np.zeros((n_items, 3), dtype=F32)
+    for i in range(n_items):
+        cx, cy = (i % nx + 0.5) * gx, (i // nx + 0.5) * gy
+        pose[i] = (cx + rng.uniform(-jitter, jitter), cy + rng.uniform(-jitter, jitter), rng.uniform(0, 2 * np.pi))
+    rad = np.array([np.max(np.hypot(p[:, 0], p[:, 1]) + p[:, 2]) for p in poles], dtype=F32)
+    pairs = candidate_pairs(pose, rad, [0.0, 0.0, width, height], depth)
+    cedges = np.array([[0, 0, width, 0], [width, 0, width, height], [width, height, 0, height],
+                       [0, height, 0, 0]], dtype=F32)
+    return dict(poles=allp, off=off, pose=pose, rad=rad, pairs=pairs, cedges=cedges)

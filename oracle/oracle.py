@@ -63,6 +63,8 @@ def lib():
         L.or_item_from_parts.restype = C.c_void_p
         L.or_item_from_parts.argtypes = [f32p, C.c_uint32, f32p, f32p, C.c_uint32, f32p, C.c_uint32, C.c_uint32]
         L.or_item_destroy.argtypes = [C.c_void_p]
+        L.or_overlap64.argtypes = [C.POINTER(C.c_double), f32p, u32p, C.c_uint32, u32p, C.c_uint64, f32p,
+                                   C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.or_query_batch.argtypes = [C.c_void_p, C.c_void_p, f32p, f32p, C.c_uint64, C.c_uint32, C.POINTER(Filter),
                                      u8p, u8p, u32p, u64p, u32p, C.c_uint32, C.c_uint32, u64p,
                                      C.POINTER(C.c_double)]
@@ -230,3 +232,18 @@ def query(cde, item, mats=None, dts=None, mode=3, filt=None, keys_cap=0, threads
     if r != 0:
         raise RuntimeError("oracle query failed (the reference would have panicked)")
     return dict(status=status, surr=surr, count=cnt, hash=hsh, keys=keys, counters=ct, seconds=sec.value)
+
+
+def overlap64(pose, poles, off, pairs, cedges, grad=True):
+    """FP64 Q and dQ/d(x, y, theta) of the pole-pair overlap quantity."""
+    pose = np.ascontiguousarray(pose, dtype=np.float64).reshape(-1, 3)
+    poles = _f(poles).reshape(-1, 3)
+    off = np.ascontiguousarray(off, dtype=np.uint32)
+    pairs = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+    cedges = _f(cedges).reshape(-1, 4)
+    q = C.c_double()
+    g = np.zeros((len(pose), 3), dtype=np.float64) if grad else None
+    lib().or_overlap64(pose.ctypes.data_as(C.POINTER(C.c_double)), fp(poles), off.ctypes.data_as(u32p), len(pose),
+                       pairs.ctypes.data_as(u32p), len(pairs), fp(cedges), len(cedges), C.byref(q),
+                       g.ctypes.data_as(C.POINTER(C.c_double)) if grad else None)
+    return q.value, g

@@ -1,0 +1,66 @@
+"""GPU pole-overlap forward and backward against the FP64 oracle (max relative errors
+asserted below 1e-4) and the backward against central differences of the GPU forward's FP64
+twin; deterministic run to run."""
+import numpy as np
+import pytest
+import torch
+
+import jagua_amd as J
+from jagua_amd import synth as S
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_errors(g, g64):
+    glob = np.linalg.norm(g - g64) / np.linalg.norm(g64)
+    big = np.abs(g64) >= 1e-3 * np.abs(g64).max()
+    comp = np.max(np.abs(g - g64)[big] / np.abs(g64)[big])
+    return glob, comp
+
+
+@pytest.mark.parametrize("n_items", [256, 4096])
+def test_overlap_vs_fp64(cuda, n_items):
+    w = S.config4_workload(seed=5, n_items=n_items, width=2000.0 * (n_items / 4096) ** 0.5,
+                           height=800.0 * (n_items / 4096) ** 0.5)
+    ov = J.PoleOverlap(w["poles"], w["off"], w["pairs"], w["cedges"])
+    pose = torch.from_numpy(w["pose"]).to(cuda)
+    q = ov.forward(pose).item()
+    g = ov.backward(pose).cpu().numpy().astype(np.float64)
+    q64, g64 = O.overlap64(w["pose"].astype(np.float64), w["poles"], w["off"], w["pairs"], w["cedges"])
+    assert q64 > 0
+    assert abs(q - q64) / q64 < 1e-4, (q, q64)
+    glob, comp = rel_errors(g, g64)
+    print(f"n_items={n_items} pairs={len(w['pairs'])} Q rel err {abs(q - q64) / q64:.2e} "
+          f"grad rel err (L2) {glob:.2e} (max comp) {comp:.2e}")
+    assert glob < 1e-4 and comp < 1e-4
+    # determinism
+    assert ov.forward(pose).item() == q
+    assert np.array_equal(ov.backward(pose).cpu().numpy().astype(np.float64), g)
+
+
+def test_overlap_backward_vs_central_differences(cuda):
+    w = S.config4_workload(seed=6, n_items=64, width=250.0, height=100.0)
+    ov = J.PoleOverlap(w["poles"], w["off"], w["pairs"], w["cedges"])
+    pose = w["pose"].astype(np.float64)
+    g = ov.backward(torch.from_numpy(w["pose"]).to(cuda)).cpu().numpy().astype(np.float64)
+    h = 1e-3
+    rng = np.random.default_rng(0)
+    idx = rng.choice(len(pose), 16, replace=False)
+    for i in idx:
+        for c in range(3):
+            p1, p2 = pose.copy(), pose.copy()
+            p1[i, c] += h
+            p2[i, c] -= h
+            fd = (O.overlap64(p1, w["poles"], w["off"], w["pairs"], w["cedges"], grad=False)[0]
+                  - O.overlap64(p2, w["poles"], w["off"], w["pairs"], w["cedges"], grad=False)[0]) / (2 * h)
+            assert abs(fd - g[i, c]) <= 1e-3 * max(1.0, abs(fd)), (i, c, fd, g[i, c])
+
+
+def test_overlap_pairs_update(cuda):
+    w = S.config4_workload(seed=7, n_items=128, width=350.0, height=140.0)
+    ov = J.PoleOverlap(w["poles"], w["off"], w["pairs"][:10], w["cedges"])
+    ov.set_pairs(w["pairs"])
+    pose = torch.from_numpy(w["pose"]).to(cuda)
+    q64, _ = O.overlap64(w["pose"].astype(np.float64), w["poles"], w["off"], w["pairs"], w["cedges"], grad=False)
+    assert abs(ov.forward(pose).item() - q64) / q64 < 1e-4
