@@ -7,8 +7,10 @@ queried by 64 items of 8-512 vertices at 16384 continuous-rotation transforms ea
 1,048,576 collision queries per step (verdict = surrogate fail-fast || polygon query, the
 composition of lbf/benches/ci_bench.rs:107-120).  Inputs are resident in HBM before timing.
 
-Prints one JSON line with the throughput, a roofline entry for the dominant kernel and the
-CPU oracle's throughput on the host cores as a reported baseline.
+One step = one pass of the batched persistent kernel over all queries.  The JSON line also
+carries the plain form (one launch per query set), the hipGraph-replayed form, the overlap
+gradient (4096 items x 16-64 poles, forward + backward per evaluation), a roofline entry for
+the dominant kernel and the CPU oracle's throughput on the host cores (a reported baseline).
 """
 import argparse
 import json
@@ -29,23 +31,38 @@ def log(msg):
 
 
 def algorithmic_bytes(counters, n_queries, n_poly, v_q, n_ff):
-    """Bytes one query must touch (transform, query geometry, nodes, hazard entries and edges,
-    containment vertices, verdict), from the oracle's traversal counters."""
+    """Bytes one batch of queries must touch: transform (24 B), ff poles (12 B each), query
+    vertices and POI when the polygon stage runs (8 B each), quadtree nodes (32 B), hazard
+    entries (32 B), hazard edges (16 B), containment polygon vertices (8 B), verdict (1 B);
+    the traversal counts come from the oracle's run of the same queries."""
     nodes, entries, edges, pip = (int(x) for x in counters)
     return (24 * n_queries + 12 * n_ff * n_queries + (8 * v_q + 8) * n_poly + 32 * nodes + 32 * entries
             + 16 * edges + 8 * pip + n_queries)
 
 
+def timed(fn, reps, stream):
+    import torch
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record(stream)
+    for _ in range(reps):
+        fn()
+    b.record(stream)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--form", choices=["batched", "plain"], default="batched")
     ap.add_argument("--items", type=int, default=64)
     ap.add_argument("--per-item", type=int, default=16384)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the oracle (CPU baseline, bit-exact check)")
+    ap.add_argument("--no-extra", action="store_true", help="only the batched form (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the query path has no "
@@ -59,6 +76,7 @@ def main():
     from jagua_amd import synth as S
 
     dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
     log("generating layout")
     scene = S.config2_scene()
     log(f"layout: {len(scene.items)} items, {len(scene.holes)} holes, {scene.n_edges} edges; building engine")
@@ -71,66 +89,26 @@ def main():
     items = S.make_items(polys)
     mats_np = [J.compose(d) for d in dts]
     total = sum(len(m) for m in mats_np)
+    offs = np.cumsum([0] + [len(m) for m in mats_np])
     mats_all = torch.from_numpy(np.concatenate(mats_np)).to(dev)
     status = torch.empty(total, dtype=torch.uint8, device=dev)
-    offs = np.cumsum([0] + [len(m) for m in mats_np])
-    stream = torch.cuda.current_stream()
-
-    plan = None
-    if args.form == "batched":
-        sets = [(cde, it, int(offs[i]), int(offs[i + 1] - offs[i]), None) for i, it in enumerate(items)]
-        plan = J.QueryPlan(sets)
+    sets = [(cde, it, int(offs[i]), int(offs[i + 1] - offs[i]), None) for i, it in enumerate(items)]
+    plan = J.QueryPlan(sets)
 
     def step():
-        if plan is not None:
-            plan.detect(mats_all, status)
-        else:
-            for i, it in enumerate(items):
-                cde.detect_batch(it, mats_all[offs[i]:offs[i + 1]], out=status[offs[i]:offs[i + 1]])
+        plan.detect(mats_all, status)
 
-    log(f"warmup {args.warmup} steps ({args.form} form, {total} queries per step)")
+    log(f"warmup {args.warmup} steps (batched form, {total} queries per step)")
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
     log(f"timing {args.steps} steps")
-    t0 = torch.cuda.Event(enable_timing=True)
-    t1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0.record(stream)
-    for _ in range(args.steps):
-        step()
-    t1.record(stream)
-    torch.cuda.synchronize()
-    ms = t0.elapsed_time(t1) / args.steps
+    ms = timed(step, args.steps, stream)
     qps = total / (ms / 1e3)
     status_np = status.cpu().numpy()
-    log(f"{ms:.3f} ms/step, {qps / 1e6:.1f} M queries/s, collision fraction {status_np.mean():.4f}")
-
-    # per-launch durations of the dominant kernel, on the stream it runs on
-    ev = []
-    if plan is not None:
-        for _ in range(5):
-            a = torch.cuda.Event(enable_timing=True)
-            b = torch.cuda.Event(enable_timing=True)
-            a.record(stream)
-            plan.detect(mats_all, status)
-            b.record(stream)
-            ev.append((a, b))
-        torch.cuda.synchronize()
-        launch_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
-        launches_per_step = 1
-    else:
-        per = []
-        for i, it in enumerate(items):
-            a = torch.cuda.Event(enable_timing=True)
-            b = torch.cuda.Event(enable_timing=True)
-            a.record(stream)
-            cde.detect_batch(it, mats_all[offs[i]:offs[i + 1]], out=status[offs[i]:offs[i + 1]])
-            b.record(stream)
-            per.append((a, b))
-        torch.cuda.synchronize()
-        launch_ms = float(np.mean([a.elapsed_time(b) for a, b in per]))
-        launches_per_step = len(items)
+    log(f"batched: {ms:.3f} ms/step, {qps / 1e6:.1f} M queries/s, collision fraction {status_np.mean():.4f}")
+    # average duration of one launch of the dominant kernel, HIP events on its stream
+    launch_ms = timed(step, 5, stream)
 
     result = {
         "metric": "bit-exact collision queries/sec on 1 MI355X, batched vs plain; overlap-grad evals/s",
@@ -145,18 +123,60 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "single large layout: 4000x1500 + 5 holes, 2000 polygons (8-256 v), depth 8; "
-                               f"{args.items} query items (8-512 v) x {args.per_item} transforms",
-                   "form": args.form, "queries_per_step": total, "layout_items": len(scene.items)},
+        "config": {"workload": "single large layout: 4000x1500 rectangle + 5 holes, 2000 polygons (8-256 "
+                               f"vertices), quadtree depth 8; {args.items} query items (8-512 vertices) x "
+                               f"{args.per_item} transforms",
+                   "form": "batched persistent kernel", "queries_per_step": total,
+                   "layout_items": len(scene.items)},
     }
 
-    # algorithmic bytes from the oracle's traversal of the same queries, and the CPU baseline
+    if not args.no_extra:
+        status_plain = torch.empty_like(status)
+
+        def plain_step():
+            for i, it in enumerate(items):
+                cde.detect_batch(it, mats_all[offs[i]:offs[i + 1]], out=status_plain[offs[i]:offs[i + 1]])
+
+        plain_step()
+        ms_plain = timed(plain_step, 3, stream)
+        plain_ok = bool(torch.equal(status_plain, status))
+        status_graph = torch.empty_like(status)
+        plan.detect_graph(mats_all, status_graph, replays=1)
+        ms_graph = timed(lambda: plan.detect_graph(mats_all, status_graph, replays=1), args.steps, stream)
+        graph_ok = bool(torch.equal(status_graph, status))
+        result["forms"] = {
+            "plain_queries_per_s": total / (ms_plain / 1e3), "plain_ms_per_step": ms_plain,
+            "batched_queries_per_s": qps, "graph_queries_per_s": total / (ms_graph / 1e3),
+            "graph_ms_per_step": ms_graph, "batched_over_plain": ms_plain / ms,
+            "plain_matches_batched": plain_ok, "graph_matches_batched": graph_ok}
+        log(f"plain: {ms_plain:.3f} ms/step; graph: {ms_graph:.3f} ms/step; same verdicts: {plain_ok and graph_ok}")
+
+        log("overlap workload: 4096 items x 16-64 poles")
+        w = S.config4_workload()
+        ov = J.PoleOverlap(w["poles"], w["off"], w["pairs"], w["cedges"])
+        pose = torch.from_numpy(w["pose"]).to(dev)
+        qbuf = torch.empty(1, dtype=torch.float32, device=dev)
+        gbuf = torch.empty((len(w["pose"]), 3), dtype=torch.float32, device=dev)
+
+        def ov_step():
+            ov.forward(pose, qbuf)
+            ov.backward(pose, gbuf)
+
+        ov_step()
+        ms_fwd = timed(lambda: ov.forward(pose, qbuf), 50, stream)
+        ms_bwd = timed(lambda: ov.backward(pose, gbuf), 50, stream)
+        ms_ov = timed(ov_step, 50, stream)
+        result["overlap"] = {"evals_per_s": 1e3 / ms_ov, "ms_per_eval": ms_ov, "fwd_ms": ms_fwd, "bwd_ms": ms_bwd,
+                             "n_items": int(len(w["pose"])), "n_poles": int(len(w["poles"])),
+                             "n_pairs": int(len(w["pairs"])), "dtype": "f32"}
+        log(f"overlap: fwd {ms_fwd:.3f} ms, bwd {ms_bwd:.3f} ms, {1e3 / ms_ov:.0f} evals/s")
+
     roof = None
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
         threads = args.cpu_threads or (os.cpu_count() or 1)
-        log(f"oracle: building engine and measuring traversal counters ({threads} threads)")
+        log(f"oracle: engine, traversal counters, CPU baseline ({threads} threads)")
         oc = O.OracleCDE(scene)
         oitems = [O.OracleItem(p) for p in polys]
         tot_bytes = 0
@@ -167,18 +187,18 @@ def main():
             n_poly = int((r["surr"] == 0).sum())
             tot_bytes += algorithmic_bytes(r["counters"], len(m), n_poly, len(polys[i]), 2)
             n_match += int((r["status"] == status_np[offs[i]:offs[i + 1]]).sum())
-            t = O.query(oc, oi, mats=m, mode=1, threads=threads)
-            cpu_sec += t["seconds"]
+            cpu_sec += min(O.query(oc, oi, mats=m, mode=1, threads=threads)["seconds"] for _ in range(2))
+        one = O.query(oc, oitems[0], mats=mats_np[0][:4096], mode=1, threads=1)["seconds"]
         log(f"oracle agreement {n_match}/{total}; CPU {total / cpu_sec / 1e6:.2f} M queries/s on {threads} threads")
-        bytes_per_launch = tot_bytes / launches_per_step
-        achieved = bytes_per_launch / (launch_ms / 1e3) / 1e9
+        achieved = tot_bytes / (launch_ms / 1e3) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "kernel": "k_detect_batched" if plan is not None else "k_detect_plain",
-                "launch_ms": launch_ms, "algorithmic_bytes_per_launch": bytes_per_launch}
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_detect_batched",
+                "launch_ms": launch_ms, "algorithmic_bytes_per_launch": tot_bytes}
         cpu = {"value": total / cpu_sec, "unit": "queries/s", "cores": threads, "kind": "port",
-               "sample": f"the full step ({total} queries, detect) on the C++ oracle, {threads} threads"}
+               "sample": f"the full step ({total} detect queries) on the C++ oracle, {threads} threads, best of 2",
+               "one_thread_queries_per_s": 4096 / one}
         result["bit_exact"] = n_match == total
+        result["vs_cpu_all_cores"] = qps / cpu["value"]
     result["roofline"] = roof
     result["cpu_baseline"] = cpu
     print(json.dumps(result), flush=True)

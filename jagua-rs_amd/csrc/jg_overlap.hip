@@ -1,10 +1,10 @@
 // Pole-pair overlap quantity and its gradient with respect to every item's (x, y, theta).
 //
 // Forward:  one wavefront per candidate item pair (lanes = poles of item a, item b's
-//           transformed poles staged in LDS) and one per item for the container edges; each
+//           rotated poles staged in LDS) and one per item for the container edges; each
 //           wave writes one partial; a two-pass fixed-order reduction (f64 accumulation)
-//           produces Q.  Penetrations follow Circle::separation_distance (circle.rs:184-200:
-//           sqrt((r1+r2)^2) - sqrt(d^2)) and the circle–edge distance (circle.rs:216-229).
+//           produces Q.  Penetrations are those of Circle::separation_distance (circle.rs:
+//           184-200, (r1 + r2) - d) and of the circle-edge distance (circle.rs:216-229).
 // Backward: the same waves recompute the pole transforms, accumulate dQ/dc per pole and
 //           chain it to (x, y, theta); per-pair and per-item partial gradients are gathered
 //           per item through a CSR adjacency in a fixed order.
@@ -30,22 +30,27 @@ __device__ __forceinline__ float wsum(float v) {
     return v;
 }
 
-// world pole: c = R(theta) p + t; also returns the rotated local offset rp = R(theta) p
-__device__ __forceinline__ void pole_world(float3 pose, float s, float c, float4 p, float2& cw, float2& rp) {
-    rp = make_float2(c * p.x - s * p.y, s * p.x + c * p.y);
-    cw = make_float2(rp.x + pose.x, rp.y + pose.y);
+// Rotated local pole offset rp = R(theta) p.  All differences are taken in the items'
+// translation frames: c_i - c_j = (t_a - t_b) + (rp_i - rp_j), where t_a - t_b is exact for
+// nearby items (Sterbenz) and the rotated offsets are small, so no precision is lost to the
+// magnitude of world coordinates.
+__device__ __forceinline__ float2 rot(float s, float c, float4 p) {
+    return make_float2(c * p.x - s * p.y, s * p.x + c * p.y);
 }
 
-// closest point on segment e to point (px, py) (edge.rs:63-87)
-__device__ __forceinline__ float2 closest_on_edge(float4 e, float px, float py) {
+// Offset from the closest point of segment e to point (px, py): the closest point is chosen
+// as in edge.rs:63-87 (clamped projection), and in the interior case the offset is formed
+// as the component perpendicular to the edge, ((a cy - b cx) / |c|^2) (cy, -cx), which is
+// the same vector without the cancellation of subtracting a far-away projected point.
+__device__ __forceinline__ float2 edge_offset(float4 e, float px, float py) {
     const float a = px - e.x, b = py - e.y, cx = e.z - e.x, cy = e.w - e.y;
-    const float dot = a * cx + b * cy;
     const float len_sq = cx * cx + cy * cy;
     float t = -1.0f;
-    if (len_sq != 0.0f) t = dot / len_sq;
-    if (t < 0.0f) return make_float2(e.x, e.y);
-    if (t > 1.0f) return make_float2(e.z, e.w);
-    return make_float2(e.x + t * cx, e.y + t * cy);
+    if (len_sq != 0.0f) t = (a * cx + b * cy) / len_sq;
+    if (t < 0.0f) return make_float2(a, b);
+    if (t > 1.0f) return make_float2(px - e.z, py - e.w);
+    const float w = (a * cy - b * cx) / len_sq;
+    return make_float2(w * cy, -(w * cx));
 }
 
 struct OvView {
@@ -69,6 +74,7 @@ __global__ __launch_bounds__(256) void k_ov_fwd_pairs(OvView v, const float* __r
     const uint2 pr = v.pairs[pid];
     const float3 pa = make_float3(pose3[3 * pr.x], pose3[3 * pr.x + 1], pose3[3 * pr.x + 2]);
     const float3 pb = make_float3(pose3[3 * pr.y], pose3[3 * pr.y + 1], pose3[3 * pr.y + 2]);
+    const float dtx = pa.x - pb.x, dty = pa.y - pb.y;
     float sa, ca, sb, cb;
     sincosf(pa.z, &sa, &ca);
     sincosf(pb.z, &sb, &cb);
@@ -78,25 +84,22 @@ __global__ __launch_bounds__(256) void k_ov_fwd_pairs(OvView v, const float* __r
     for (uint32_t jb = 0; jb < nb; jb += 64) {
         const uint32_t jn = min(64u, nb - jb);
         if (lane < jn) {
-            float2 cw, rp;
             const float4 p = v.poles[b0 + jb + lane];
-            pole_world(pb, sb, cb, p, cw, rp);
-            lds[wave][lane] = make_float4(cw.x, cw.y, p.z, 0.0f);
+            const float2 rp = rot(sb, cb, p);
+            lds[wave][lane] = make_float4(rp.x, rp.y, p.z, 0.0f);
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         for (uint32_t i = lane; i < na; i += 64) {
-            float2 ci, rpi;
             const float4 p = v.poles[a0 + i];
-            pole_world(pa, sa, ca, p, ci, rpi);
+            const float2 rpi = rot(sa, ca, p);
             for (uint32_t j = 0; j < jn; j++) {
                 const float4 q = lds[wave][j];
-                const float dx = ci.x - q.x, dy = ci.y - q.y;
+                const float dx = (rpi.x - q.x) + dtx, dy = (rpi.y - q.y) + dty;
                 const float sq = dx * dx + dy * dy;
                 const float rs = p.z + q.z;
-                const float sqr = rs * rs;
-                if (sq < sqr) {
-                    const float pen = sqrtf(sqr) - sqrtf(sq);
+                if (sq < rs * rs) {
+                    const float pen = rs - sqrtf(sq);
                     acc += pen * pen;
                 }
             }
@@ -119,13 +122,13 @@ __global__ __launch_bounds__(256) void k_ov_fwd_container(OvView v, const float*
     const uint32_t a0 = v.off[item], na = v.off[item + 1] - a0;
     float acc = 0.0f;
     for (uint32_t i = lane; i < na; i += 64) {
-        float2 ci, rp;
         const float4 p = v.poles[a0 + i];
-        pole_world(pa, s, c, p, ci, rp);
+        const float2 rp = rot(s, c, p);
         for (uint32_t e = 0; e < v.n_cedges; e++) {
-            const float4 ed = v.cedges[e];
-            const float2 q = closest_on_edge(ed, ci.x, ci.y);
-            const float dx = ci.x - q.x, dy = ci.y - q.y;
+            const float4 ew = v.cedges[e];
+            const float4 ed = make_float4(ew.x - pa.x, ew.y - pa.y, ew.z - pa.x, ew.w - pa.y);
+            const float2 o = edge_offset(ed, rp.x, rp.y);
+            const float dx = o.x, dy = o.y;
             const float d = sqrtf(dx * dx + dy * dy);
             if (d < p.z) {
                 const float pen = p.z - d;
@@ -171,14 +174,14 @@ __global__ __launch_bounds__(256) void k_ov_reduce2(const double* __restrict__ x
 // out[pid] = (dQ/dx_a, dQ/dy_a, dQ/dtheta_a, dQ/dx_b, dQ/dy_b, dQ/dtheta_b)
 __global__ __launch_bounds__(256) void k_ov_bwd_pairs(OvView v, const float* __restrict__ pose3,
                                                       float* __restrict__ out6) {
-    __shared__ float4 lds_c[OV_WAVES][64];
-    __shared__ float2 lds_r[OV_WAVES][64];
+    __shared__ float4 lds[OV_WAVES][64];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint64_t pid = (uint64_t)blockIdx.x * OV_WAVES + wave;
     if (pid >= v.n_pairs) return;
     const uint2 pr = v.pairs[pid];
     const float3 pa = make_float3(pose3[3 * pr.x], pose3[3 * pr.x + 1], pose3[3 * pr.x + 2]);
     const float3 pb = make_float3(pose3[3 * pr.y], pose3[3 * pr.y + 1], pose3[3 * pr.y + 2]);
+    const float dtx = pa.x - pb.x, dty = pa.y - pb.y;
     float sa, ca, sb, cb;
     sincosf(pa.z, &sa, &ca);
     sincosf(pb.z, &sb, &cb);
@@ -188,36 +191,30 @@ __global__ __launch_bounds__(256) void k_ov_bwd_pairs(OvView v, const float* __r
     for (uint32_t jb = 0; jb < nb; jb += 64) {
         const uint32_t jn = min(64u, nb - jb);
         if (lane < jn) {
-            float2 cw, rp;
             const float4 p = v.poles[b0 + jb + lane];
-            pole_world(pb, sb, cb, p, cw, rp);
-            lds_c[wave][lane] = make_float4(cw.x, cw.y, p.z, 0.0f);
-            lds_r[wave][lane] = rp;
+            const float2 rp = rot(sb, cb, p);
+            lds[wave][lane] = make_float4(rp.x, rp.y, p.z, 0.0f);
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         for (uint32_t i = lane; i < na; i += 64) {
-            float2 ci, rpi;
             const float4 p = v.poles[a0 + i];
-            pole_world(pa, sa, ca, p, ci, rpi);
+            const float2 rpi = rot(sa, ca, p);
             float gcx = 0.0f, gcy = 0.0f;
             for (uint32_t j = 0; j < jn; j++) {
-                const float4 q = lds_c[wave][j];
-                const float dx = ci.x - q.x, dy = ci.y - q.y;
+                const float4 q = lds[wave][j];
+                const float dx = (rpi.x - q.x) + dtx, dy = (rpi.y - q.y) + dty;
                 const float sq = dx * dx + dy * dy;
                 const float rs = p.z + q.z;
-                const float sqr = rs * rs;
-                if (sq < sqr && sq > 0.0f) {
+                if (sq < rs * rs && sq > 0.0f) {
                     const float d = sqrtf(sq);
-                    const float pen = sqrtf(sqr) - d;
-                    const float k = -2.0f * pen / d;        // dQ/dc_i = k (c_i - c_j)
+                    const float k = -2.0f * (rs - d) / d;   // dQ/dc_i = k (c_i - c_j) = -dQ/dc_j
                     const float gx = k * dx, gy = k * dy;
                     gcx += gx;
                     gcy += gy;
-                    const float2 rpj = lds_r[wave][j];      // dc_j/dtheta_b = (-rp_j.y, rp_j.x)
-                    gxb -= gx;
+                    gxb -= gx;                              // dc_j/dtheta_b = (-rp_j.y, rp_j.x)
                     gyb -= gy;
-                    gtb += gx * rpj.y - gy * rpj.x;
+                    gtb += gx * q.y - gy * q.x;
                 }
             }
             gxa += gcx;
@@ -246,15 +243,15 @@ __global__ __launch_bounds__(256) void k_ov_bwd_container(OvView v, const float*
     const uint32_t a0 = v.off[item], na = v.off[item + 1] - a0;
     float gx = 0, gy = 0, gt = 0;
     for (uint32_t i = lane; i < na; i += 64) {
-        float2 ci, rp;
         const float4 p = v.poles[a0 + i];
-        pole_world(pa, s, c, p, ci, rp);
+        const float2 rp = rot(s, c, p);
         float gcx = 0.0f, gcy = 0.0f;
         for (uint32_t e = 0; e < v.n_cedges; e++) {
-            const float2 q = closest_on_edge(v.cedges[e], ci.x, ci.y);
-            const float dx = ci.x - q.x, dy = ci.y - q.y;
-            const float sq = dx * dx + dy * dy;
-            const float d = sqrtf(sq);
+            const float4 ew = v.cedges[e];
+            const float4 ed = make_float4(ew.x - pa.x, ew.y - pa.y, ew.z - pa.x, ew.w - pa.y);
+            const float2 o = edge_offset(ed, rp.x, rp.y);
+            const float dx = o.x, dy = o.y;
+            const float d = sqrtf(dx * dx + dy * dy);
             if (d < p.z && d > 0.0f) {
                 const float k = -2.0f * (p.z - d) / d;
                 gcx += k * dx;
