@@ -104,9 +104,9 @@ class SPolygon:
         check(N.lib().jg_spolygon_new(_ptr(pts), pts.shape[0], C.byref(h)))
         return cls(h.value)
 
-    def __del__(self):
-        if getattr(self, "_h", None) and self._h.value and N._lib is not None:
-            N._lib.jg_spolygon_free(self._h)
+    def __del__(self, _N=N):
+        if getattr(self, "_h", None) and self._h.value and _N._lib is not None:
+            _N._lib.jg_spolygon_free(self._h)
             self._h = C.c_void_p()
 
     def generate_surrogate(self, config=None):
@@ -283,9 +283,9 @@ class CDESnapshot:
     def __init__(self, handle):
         self._h = C.c_void_p(handle)
 
-    def __del__(self):
-        if getattr(self, "_h", None) and self._h.value and N._lib is not None:
-            N._lib.jg_snapshot_free(self._h)
+    def __del__(self, _N=N):
+        if getattr(self, "_h", None) and self._h.value and _N._lib is not None:
+            _N._lib.jg_snapshot_free(self._h)
             self._h = C.c_void_p()
 
 
@@ -318,9 +318,9 @@ class CDEngine:
         cfg = self.config._c()
         check(N.lib().jg_cde_new(_ptr(bb), C.byref(cfg), ents, shapes, dyn, n, C.byref(self._h)))
 
-    def __del__(self):
-        if getattr(self, "_h", None) and self._h.value and N._lib is not None:
-            N._lib.jg_cde_free(self._h)
+    def __del__(self, _N=N):
+        if getattr(self, "_h", None) and self._h.value and _N._lib is not None:
+            _N._lib.jg_cde_free(self._h)
             self._h = C.c_void_p()
 
     def register_hazard(self, hazard):
@@ -436,9 +436,9 @@ class QueryPlan:
         self._refs = [s[0] for s in sets] + [s[1] for s in sets]
         check(N.lib().jg_plan_create(arr, n, C.byref(cfg), C.byref(self._h)))
 
-    def __del__(self):
-        if getattr(self, "_h", None) and self._h.value and N._lib is not None:
-            N._lib.jg_plan_free(self._h)
+    def __del__(self, _N=N):
+        if getattr(self, "_h", None) and self._h.value and _N._lib is not None:
+            _N._lib.jg_plan_free(self._h)
             self._h = C.c_void_p()
 
     def detect(self, mats, out, stream=None):
@@ -469,9 +469,9 @@ class PoleOverlap:
         check(N.lib().jg_overlap_create(_ptr(poles), _ptr(off, C.c_uint32), self.n_items, _ptr(pr, C.c_uint32), len(pr),
                                         _ptr(ce), len(ce), C.byref(self._h)))
 
-    def __del__(self):
-        if getattr(self, "_h", None) and self._h.value and N._lib is not None:
-            N._lib.jg_overlap_free(self._h)
+    def __del__(self, _N=N):
+        if getattr(self, "_h", None) and self._h.value and _N._lib is not None:
+            _N._lib.jg_overlap_free(self._h)
             self._h = C.c_void_p()
 
     def set_pairs(self, pairs):
