@@ -113,8 +113,7 @@ jg_status cde_upload(jg_cde* c) {
         off = align256(off + bytes);
         return o;
     };
-    const size_t o_nb = place(sizeof(float4) * f.node_bbox.size());
-    const size_t o_ni = place(sizeof(int4) * f.node_info.size());
+    const size_t o_nd = place(sizeof(FlatNode) * f.nodes.size());
     const size_t o_nu = place(sizeof(uint32_t) * f.node_up.size());
     const size_t o_en = place(sizeof(FlatEntry) * f.ent.size());
     const size_t o_ed = place(sizeof(float4) * (f.edges.size() + 1));
@@ -127,8 +126,7 @@ jg_status cde_upload(jg_cde* c) {
     auto put = [&](size_t o, const void* src, size_t bytes) {
         if (bytes) std::memcpy(host.data() + o, src, bytes);
     };
-    put(o_nb, f.node_bbox.data(), sizeof(float4) * f.node_bbox.size());
-    put(o_ni, f.node_info.data(), sizeof(int4) * f.node_info.size());
+    put(o_nd, f.nodes.data(), sizeof(FlatNode) * f.nodes.size());
     put(o_nu, f.node_up.data(), sizeof(uint32_t) * f.node_up.size());
     put(o_en, f.ent.data(), sizeof(FlatEntry) * f.ent.size());
     put(o_ed, f.edges.data(), sizeof(float4) * f.edges.size());
@@ -143,8 +141,7 @@ jg_status cde_upload(jg_cde* c) {
     auto t2 = std::chrono::steady_clock::now();
     uint8_t* b = (uint8_t*)c->dev.ptr;
     CdeView& v = c->view;
-    v.node_bbox = (const float4*)(b + o_nb);
-    v.node_info = (const int4*)(b + o_ni);
+    v.nodes = (const DevNode*)(b + o_nd);
     v.node_up = (const uint32_t*)(b + o_nu);
     v.ent = (const DevEntry*)(b + o_en);
     v.edges = (const float4*)(b + o_ed);
@@ -155,8 +152,8 @@ jg_status cde_upload(jg_cde* c) {
     v.n_slots = (uint32_t)f.haz_bbox.size();
     v.exterior_slot = c->exterior.idx;
     v.cd_threshold = c->cfg.cd_threshold;
-    v.n_nodes = (uint32_t)f.node_bbox.size();
-    c->stats.n_nodes = f.node_bbox.size();
+    v.n_nodes = (uint32_t)f.nodes.size();
+    c->stats.n_nodes = f.nodes.size();
     c->stats.n_entries = f.ent.size();
     c->stats.n_edge_refs = f.edges.size();
     c->stats.n_hazard_vertices = f.haz_verts.size();
@@ -221,7 +218,7 @@ __global__ __launch_bounds__(256) void k_detect_plain(CdeView c, ItemView it, co
     }
     if (!(flags & JG_QUERY_POLY)) { status[q] = JG_Q_CLEAR; return; }
     const float4 qbb = query_bbox(it, m);
-    if (box_relation(c.node_bbox[0], qbb) != REL_SURROUNDING) { status[q] = JG_Q_COLLISION; return; }
+    if (box_relation(c.nodes[0].bbox, qbb) != REL_SURROUNDING) { status[q] = JG_Q_COLLISION; return; }
     const int vr = virtual_root(c, qbb);
     float2 a = xform(m, it.v[0].x, it.v[0].y);
     const float2 first = a;
@@ -230,9 +227,10 @@ __global__ __launch_bounds__(256) void k_detect_plain(CdeView c, ItemView it, co
         if (collides_edge(c, vr, make_float4(a.x, a.y, b.x, b.y), f, q)) { status[q] = JG_Q_COLLISION; return; }
         a = b;
     }
-    const int4 info = c.node_info[vr];
+    const DevNode vn = c.nodes[vr];
     const float2 qpoi = xform(m, it.poi.x, it.poi.y);
-    for (int i = info.y; i < info.y + info.z; i++) {
+    const int he = (int)(vn.haz_begin + (vn.haz_info & COUNT_MASK));
+    for (int i = (int)vn.haz_begin; i < he; i++) {
         const uint32_t sf = c.ent[i].slot_flags;
         if (sf & ENTIRE_BIT) { status[q] = JG_Q_PANIC_FILTERED_ENTIRE; return; }
         const uint32_t slot = sf & SLOT_MASK;
@@ -250,8 +248,10 @@ __device__ __forceinline__ void set_put(uint32_t* set, uint32_t slot) { atomicOr
 // Hazard tests of QTNode::collect_collisions at a node where detection is performed now
 // (qt_node.rs:174-189), for a circle or an edge.
 template <bool CIRCLE>
-__device__ __forceinline__ void collect_now(const CdeView& c, int4 info, float4 ent_geom, float r, uint32_t* set) {
-    for (int i = info.y; i < info.y + info.z; i++) {
+__device__ __forceinline__ void collect_now(const CdeView& c, const DevNode& nd, float4 ent_geom, float r,
+                                            uint32_t* set) {
+    const int he = (int)(nd.haz_begin + (nd.haz_info & COUNT_MASK));
+    for (int i = (int)nd.haz_begin; i < he; i++) {
         const DevEntry e = c.ent[i];
         const uint32_t slot = e.slot_flags & SLOT_MASK;
         if (set_has(set, slot)) continue;
@@ -272,11 +272,11 @@ __device__ __forceinline__ void collect_now(const CdeView& c, int4 info, float4 
 template <bool CIRCLE>
 __device__ __forceinline__ void collect_from(const CdeView& c, int start, float4 g, float r, uint32_t* set) {
     descend(c, start, [&](int n, int& fc) -> uint32_t {
-        const int4 info = c.node_info[n];
-        const bool now = (uint32_t)info.w <= c.cd_threshold;
-        if (info.x >= 0 && !now) {
-            fc = info.x;
-            const float4 nb = c.node_bbox[n];
+        const DevNode nd = c.nodes[n];
+        const bool now = nd.n_active_edges <= c.cd_threshold;
+        if (nd.first_child >= 0 && !now) {
+            fc = nd.first_child;
+            const float4 nb = nd.bbox;
             if (CIRCLE) {
                 uint32_t m = 0;
                 for (int k = 0; k < 4; k++) m |= circle_box(g.x, g.y, r, box_quadrant(nb, k)) ? (1u << k) : 0u;
@@ -286,7 +286,7 @@ __device__ __forceinline__ void collect_from(const CdeView& c, int start, float4
             for (int k = 0; k < 4; k++) qs[k] = box_quadrant(nb, k);
             return edge_quadrants(g, nb, qs);
         }
-        collect_now<CIRCLE>(c, info, g, r, set);
+        collect_now<CIRCLE>(c, nd, g, r, set);
         return 0u;
     });
 }
@@ -349,7 +349,7 @@ __global__ __launch_bounds__(256) void k_collect(CdeView c, ItemView it, const f
             bb.x = mn(bb.x, p.x); bb.y = mn(bb.y, p.y); bb.z = mx(bb.z, p.x); bb.w = mx(bb.w, p.y);
         }
         qbb = make_float4(wave_min(bb.x), wave_min(bb.y), wave_max(bb.z), wave_max(bb.w));
-        if (box_relation(c.node_bbox[0], qbb) != REL_SURROUNDING && lane == 0) set_put(set, c.exterior_slot);
+        if (box_relation(c.nodes[0].bbox, qbb) != REL_SURROUNDING && lane == 0) set_put(set, c.exterior_slot);
         vr = virtual_root(c, qbb);
         for (uint32_t i = lane; i < it.n_v; i += 64) {
             const float2 a = xform(m, it.v[i].x, it.v[i].y);
@@ -360,9 +360,10 @@ __global__ __launch_bounds__(256) void k_collect(CdeView c, ItemView it, const f
     }
     __syncthreads();
     if (active) {
-        const int4 info = c.node_info[vr];
+        const DevNode vn = c.nodes[vr];
         const float2 qpoi = xform(m, it.poi.x, it.poi.y);
-        for (int i = info.y + (int)lane; i < info.y + info.z; i += 64) {
+        const int he = (int)(vn.haz_begin + (vn.haz_info & COUNT_MASK));
+        for (int i = (int)vn.haz_begin + (int)lane; i < he; i += 64) {
             const uint32_t sf = c.ent[i].slot_flags;
             if (sf & ENTIRE_BIT) continue;
             const uint32_t slot = sf & SLOT_MASK;
@@ -612,21 +613,21 @@ extern "C" jg_status jg_cde_dump(const jg_cde* c, uint32_t* nodes, uint64_t node
     if (!c || !n_nodes || !n_haz) return fail(JG_ERR_INVALID_ARG, "null argument");
     FlatCDE f;
     flatten(c->qt, c->map, f);
-    *n_nodes = f.node_bbox.size();
+    *n_nodes = f.nodes.size();
     *n_haz = f.ent.size();
-    for (size_t i = 0; nodes && i < f.node_bbox.size() && i < nodes_cap; i++) {
+    for (size_t i = 0; nodes && i < f.nodes.size() && i < nodes_cap; i++) {
         uint32_t* o = nodes + 8 * i;
         // level is recovered from the BFS structure: root level = depth, children one less
         o[0] = 0;
-        o[1] = f.node_info[i].x >= 0 ? 4 : 0;
-        o[2] = (uint32_t)f.node_info[i].z;
-        o[3] = (uint32_t)f.node_info[i].w;
-        std::memcpy(o + 4, &f.node_bbox[i], 16);
+        o[1] = f.nodes[i].first_child >= 0 ? 4 : 0;
+        o[2] = f.nodes[i].haz_info & 0x3fffffffu;
+        o[3] = f.nodes[i].n_active_edges;
+        std::memcpy(o + 4, &f.nodes[i].bbox, 16);
     }
     if (nodes) {   // fill levels top-down
-        for (size_t i = 0; i < f.node_bbox.size() && i < nodes_cap; i++) {
+        for (size_t i = 0; i < f.nodes.size() && i < nodes_cap; i++) {
             if (i == 0) nodes[0] = c->cfg.quadtree_depth;
-            const int fc = f.node_info[i].x;
+            const int fc = f.nodes[i].first_child;
             if (fc >= 0)
                 for (int k = 0; k < 4; k++)
                     if ((uint64_t)(fc + k) < nodes_cap) nodes[8 * (size_t)(fc + k)] = nodes[8 * i] - 1;

@@ -1,12 +1,17 @@
 // Batched persistent query form (jg_plan_*): many (engine, item, transform range) sets run
-// in one launch.  Work units of up to 64 queries are pulled from a device queue by
-// wavefronts:
-//   1. surrogate fail-fast (cd_engine.rs:204-223): lanes = (query, ff pole/pier) pairs, each
-//      lane descends from the root; a ballot yields the chunk's surviving queries;
-//   2. polygon query (cd_engine.rs:160-197) per survivor: lanes = query edges, descending from
-//      the virtual root, with a wave-wide early exit after every 64 edges;
-//   3. containment (cd_engine.rs:176-193): lanes = the virtual root's hazard entries.
-// Units are ordered by estimated cost (largest items first) so the tail stays short.
+// in one launch sequence fed by device work queues.
+//   k_surrogate: work units of up to 64 queries; lanes = (query, ff pole/pier) pairs, each
+//                lane descends from the root (cd_engine.rs:204-223).  Collisions are written
+//                out; every unit with survivors appends one (set, first query, 64-bit mask)
+//                group to the survivor queue.
+//   k_poly:      survivor groups; each query is handled by G lanes (G = the item's vertex
+//                count rounded up to a power of two, 8..64), 64/G queries per wavefront:
+//                transformed bbox, virtual root, the item's edges descending from it with a
+//                group-wide early exit, then containment over the virtual root's hazards
+//                (cd_engine.rs:160-197).
+// Splitting the two stages keeps the light surrogate kernel at high occupancy and lets the
+// polygon stage pack several small items per wavefront.  Units are ordered by estimated
+// cost (largest items first) so the tail stays short.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,43 +34,46 @@ struct SetDesc {
     ItemView item;
     FilterView filter;
     uint64_t first;
-    uint32_t flags, pad;
+    uint32_t flags;
+    uint32_t lanes;        // lanes per query in the polygon stage (power of two, 1..64)
 };
 
-struct Unit {             // 16 B
+struct Unit {              // 16 B
     uint32_t set;
-    uint32_t q0;          // first query, relative to the set
-    uint32_t count;       // <= 64
+    uint32_t q0;           // first query, relative to the set
+    uint32_t count;        // <= 64
     uint32_t pad;
 };
 
-constexpr int WAVE = 64;
+struct Group {             // 16 B: surviving queries q0 + i for the set bits i of mask
+    uint32_t set;
+    uint32_t q0;
+    uint64_t mask;
+};
+
+// counters: [0] surrogate units taken, [1] groups appended, [2] groups taken
+constexpr int C_UNITS = 0, C_GROUPS = 1, C_TAKEN = 2;
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
-__device__ __forceinline__ float wmin(float v) {
-    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ float wmax(float v) {
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
-}
+template <typename T>
+__device__ __forceinline__ T wave_bcast(T v) { return (T)__shfl((int)v, 0, 64); }
 
 }  // namespace
 
-__global__ void k_reset_counter(uint32_t* counter) {
-    if (threadIdx.x == 0) *counter = 0u;
+__global__ void k_reset_counters(uint32_t* counters) {
+    if (threadIdx.x < 4) counters[threadIdx.x] = 0u;
 }
 
-__global__ __launch_bounds__(256) void k_detect_batched(const SetDesc* __restrict__ sets, const Unit* __restrict__ units,
-                                                        uint32_t n_units, uint32_t* __restrict__ counter,
-                                                        const float* __restrict__ m6, uint8_t* __restrict__ status) {
+__global__ __launch_bounds__(256) void k_surrogate(const SetDesc* __restrict__ sets, const Unit* __restrict__ units,
+                                                   uint32_t n_units, uint32_t* __restrict__ counters,
+                                                   Group* __restrict__ groups, const float* __restrict__ m6,
+                                                   uint8_t* __restrict__ status) {
     const uint32_t lane = threadIdx.x & 63u;
     for (;;) {
         uint32_t u = 0;
-        if (lane == 0) u = atomicAdd(counter, 1u);
-        u = (uint32_t)__shfl((int)u, 0, 64);
+        if (lane == 0) u = atomicAdd(&counters[C_UNITS], 1u);
+        u = wave_bcast(u);
         if (u >= n_units) return;
         const Unit un = units[u];
         const SetDesc& sd = sets[un.set];
@@ -73,22 +81,22 @@ __global__ __launch_bounds__(256) void k_detect_batched(const SetDesc* __restric
         const ItemView it = sd.item;
         const FilterView f = sd.filter;
         const uint32_t flags = sd.flags;
-        const uint64_t base = sd.first;
+        const uint64_t base = sd.first + un.q0;
         const uint32_t cnt = un.count;
+        const uint64_t valid = cnt == 64 ? ~0ull : ((1ull << cnt) - 1ull);
 
-        // ---- phase 1: surrogate, lanes = (query, entity)
-        uint64_t hit_mask = 0;                        // bit i: query q0+i collided in the surrogate
+        uint64_t hit_mask = 0;
         const uint32_t n_ent = (flags & JG_QUERY_SURROGATE) ? it.n_poles + it.n_piers : 0u;
         if (n_ent > 0) {
             uint32_t g = 1;
             while (g < n_ent && g < 64u) g <<= 1;     // lanes per query
-            const uint32_t qpi = 64u / g;             // queries per pass
-            for (uint32_t b = 0; b < cnt; b += qpi) {
+            const uint32_t qpp = 64u / g;             // queries per pass
+            for (uint32_t b = 0; b < cnt; b += qpp) {
                 const uint32_t qi = b + lane / g;
                 bool hit = false;
                 if (qi < cnt) {
                     const uint64_t q = (uint64_t)un.q0 + qi;
-                    const Mat m = load_mat(m6, base + q);
+                    const Mat m = load_mat(m6, sd.first + q);
                     for (uint32_t e = lane % g; e < n_ent && !hit; e += g) {
                         const float4 p = it.ff[e];
                         if (e < it.n_poles) {
@@ -101,70 +109,113 @@ __global__ __launch_bounds__(256) void k_detect_batched(const SetDesc* __restric
                     }
                 }
                 const uint64_t bal = ballot(hit);
-                // fold each group of g lanes into one bit per query
-                for (uint32_t j = 0; j < qpi && b + j < cnt; j++) {
+                for (uint32_t j = 0; j < qpp && b + j < cnt; j++) {
                     const uint64_t grp = g == 64 ? ~0ull : (((1ull << g) - 1ull) << (j * g));
                     if (bal & grp) hit_mask |= 1ull << (b + j);
                 }
             }
         }
+        const uint64_t todo = (flags & JG_QUERY_POLY) ? (valid & ~hit_mask) : 0ull;
+        if (lane < cnt) {
+            const bool h = (hit_mask >> lane) & 1ull;
+            if (h) status[base + lane] = JG_Q_COLLISION;
+            else if (!(flags & JG_QUERY_POLY)) status[base + lane] = JG_Q_CLEAR;
+        }
+        if (todo != 0ull && lane == 0) {
+            const uint32_t gi = atomicAdd(&counters[C_GROUPS], 1u);
+            groups[gi] = Group{un.set, un.q0, todo};
+        }
+    }
+}
 
-        // ---- phase 2 + 3: polygon query per surviving query, lanes = edges
-        uint8_t my_status = JG_Q_COLLISION;           // lane l reports query q0 + l
-        uint64_t todo = (cnt == 64 ? ~0ull : ((1ull << cnt) - 1ull)) & ~hit_mask;
-        if (!(flags & JG_QUERY_POLY)) {
-            my_status = (hit_mask >> lane) & 1ull ? JG_Q_COLLISION : JG_Q_CLEAR;
-            todo = 0;
-        }
+__global__ __launch_bounds__(256) void k_poly(const SetDesc* __restrict__ sets, uint32_t* __restrict__ counters,
+                                              const Group* __restrict__ groups, const float* __restrict__ m6,
+                                              uint8_t* __restrict__ status) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t n_groups = __atomic_load_n(&counters[C_GROUPS], __ATOMIC_RELAXED);
+    for (;;) {
+        uint32_t gi = 0;
+        if (lane == 0) gi = atomicAdd(&counters[C_TAKEN], 1u);
+        gi = wave_bcast(gi);
+        if (gi >= n_groups) return;
+        const Group gr = groups[gi];
+        const SetDesc& sd = sets[gr.set];
+        const CdeView c = sd.cde;
+        const ItemView it = sd.item;
+        const FilterView f = sd.filter;
+        const uint32_t G = sd.lanes;
+        const uint32_t per = 64u / G;                     // queries per pass
+        const uint32_t slot = lane / G, sub = lane % G;
+        const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+        const float4 root_bb = c.nodes[0].bbox;
+        uint64_t todo = gr.mask;
         while (todo) {
-            const uint32_t qi = (uint32_t)__builtin_ctzll(todo);
-            todo &= todo - 1;
-            const uint64_t q = (uint64_t)un.q0 + qi;
-            const Mat m = load_mat(m6, base + q);
+            // this lane's query: the slot-th remaining survivor
+            uint64_t t = todo;
+            for (uint32_t s = 0; s < slot && t; s++) t &= t - 1;
+            const bool active = t != 0ull;
+            const uint32_t qi = active ? (uint32_t)__builtin_ctzll(t) : 0u;
+            for (uint32_t s = 0; s < per && todo; s++) todo &= todo - 1;
+            const uint64_t q = (uint64_t)gr.q0 + qi;
+            Mat m{};
+            if (active) m = load_mat(m6, sd.first + q);
+            // transformed bbox (simple_polygon.rs:122-133), reduced over the group's lanes
             float4 bb = make_float4(3.40282347e+38f, 3.40282347e+38f, -3.40282347e+38f, -3.40282347e+38f);
-            for (uint32_t i = lane; i < it.n_v; i += WAVE) {
-                const float2 p = xform(m, it.v[i].x, it.v[i].y);
-                bb.x = mn(bb.x, p.x); bb.y = mn(bb.y, p.y); bb.z = mx(bb.z, p.x); bb.w = mx(bb.w, p.y);
+            if (active)
+                for (uint32_t i = sub; i < it.n_v; i += G) {
+                    const float2 p = xform(m, it.v[i].x, it.v[i].y);
+                    bb.x = mn(bb.x, p.x); bb.y = mn(bb.y, p.y); bb.z = mx(bb.z, p.x); bb.w = mx(bb.w, p.y);
+                }
+            for (uint32_t o = G >> 1; o > 0; o >>= 1) {
+                bb.x = fminf(bb.x, __shfl_xor(bb.x, (int)o, 64));
+                bb.y = fminf(bb.y, __shfl_xor(bb.y, (int)o, 64));
+                bb.z = fmaxf(bb.z, __shfl_xor(bb.z, (int)o, 64));
+                bb.w = fmaxf(bb.w, __shfl_xor(bb.w, (int)o, 64));
             }
-            const float4 qbb = make_float4(wmin(bb.x), wmin(bb.y), wmax(bb.z), wmax(bb.w));
-            uint8_t res;
-            if (box_relation(c.node_bbox[0], qbb) != REL_SURROUNDING) {
+            uint8_t res = JG_Q_CLEAR;
+            bool done = !active;
+            if (active && box_relation(root_bb, bb) != REL_SURROUNDING) {
                 res = JG_Q_COLLISION;
-            } else {
-                const int vr = virtual_root(c, qbb);
-                bool any = false;
-                for (uint32_t i0 = 0; i0 < it.n_v && !any; i0 += WAVE) {
-                    const uint32_t i = i0 + lane;
-                    bool hit = false;
-                    if (i < it.n_v) {
-                        const uint32_t j = i + 1 == it.n_v ? 0 : i + 1;
-                        const float2 a = xform(m, it.v[i].x, it.v[i].y);
-                        const float2 b2 = xform(m, it.v[j].x, it.v[j].y);
-                        hit = collides_edge(c, vr, make_float4(a.x, a.y, b2.x, b2.y), f, q);
-                    }
-                    any = ballot(hit) != 0ull;
+                done = true;
+            }
+            const int vr = active ? virtual_root(c, bb) : 0;
+            for (uint32_t i0 = 0; i0 < it.n_v; i0 += G) {
+                if (ballot(!done) == 0ull) break;
+                const uint32_t i = i0 + sub;
+                bool hit = false;
+                if (!done && i < it.n_v) {
+                    const uint32_t j = i + 1 == it.n_v ? 0 : i + 1;
+                    const float2 a = xform(m, it.v[i].x, it.v[i].y);
+                    const float2 b2 = xform(m, it.v[j].x, it.v[j].y);
+                    hit = collides_edge(c, vr, make_float4(a.x, a.y, b2.x, b2.y), f, q);
                 }
-                if (any) {
+                const uint64_t bal = ballot(hit);
+                if (!done && ((bal >> (slot * G)) & gmask)) {
                     res = JG_Q_COLLISION;
+                    done = true;
+                }
+            }
+            // containment over the virtual root's hazards (cd_engine.rs:176-193)
+            bool hit = false;
+            if (!done) {
+                const DevNode vn = c.nodes[vr];
+                const uint32_t cntv = vn.haz_info & COUNT_MASK;
+                if (cntv > 0 && (vn.haz_info & ENTIRE_BIT)) {
+                    res = JG_Q_PANIC_FILTERED_ENTIRE;
+                    done = true;
                 } else {
-                    const int4 info = c.node_info[vr];
-                    if (info.z > 0 && (c.ent[info.y].slot_flags & ENTIRE_BIT)) {
-                        res = JG_Q_PANIC_FILTERED_ENTIRE;
-                    } else {
-                        const float2 qpoi = xform(m, it.poi.x, it.poi.y);
-                        bool hit = false;
-                        for (int i = info.y + (int)lane; i < info.y + info.z; i += WAVE) {
-                            const uint32_t slot = c.ent[i].slot_flags & SLOT_MASK;
-                            if (irrelevant(f, q, slot)) continue;
-                            if (containment(c, slot, it, m, qbb, qpoi)) { hit = true; break; }
-                        }
-                        res = ballot(hit) != 0ull ? JG_Q_COLLISION : JG_Q_CLEAR;
+                    const float2 qpoi = xform(m, it.poi.x, it.poi.y);
+                    for (uint32_t i = sub; i < cntv; i += G) {
+                        const uint32_t slot_h = c.ent[vn.haz_begin + i].slot_flags & SLOT_MASK;
+                        if (irrelevant(f, q, slot_h)) continue;
+                        if (containment(c, slot_h, it, m, bb, qpoi)) { hit = true; break; }
                     }
                 }
             }
-            if (lane == qi) my_status = res;
+            const uint64_t bal = ballot(hit);
+            if (!done) res = ((bal >> (slot * G)) & gmask) ? JG_Q_COLLISION : JG_Q_CLEAR;
+            if (active && sub == 0) status[sd.first + q] = res;
         }
-        if (lane < cnt) status[base + un.q0 + lane] = my_status;
     }
 }
 
@@ -174,12 +225,12 @@ struct jg_plan {
     std::vector<jg_hazard_filter> filters;
     std::vector<std::vector<jg_hazkey>> filter_keys;
     jg_plan_config cfg{};
-    DevBlob table;                  // SetDesc[] then Unit[] then the counter
+    DevBlob table;                  // SetDesc[], Unit[], counters, survivor groups
     DevBlob masks;                  // JG_FILTER_KEYS masks of every set
     std::vector<uint64_t> generations;
     uint32_t n_units = 0;
-    size_t off_units = 0, off_counter = 0;
-    uint32_t grid = 0;
+    size_t off_units = 0, off_counter = 0, off_groups = 0;
+    uint32_t grid_s = 0, grid_p = 0;
     // graph cache
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -234,6 +285,14 @@ jg_status plan_build_table(jg_plan* p) {
         sd[i].flags = p->cfg.flags ? p->cfg.flags : (JG_QUERY_SURROGATE | JG_QUERY_POLY);
         if (!(sd[i].flags & JG_QUERY_SURROGATE)) { sd[i].item.n_poles = 0; sd[i].item.n_piers = 0; }
         sd[i].first = s.first;
+        uint32_t lanes = p->cfg.lanes_per_query;
+        if (lanes == 0) {
+            lanes = 8;
+            while (lanes < s.item->poly.n() && lanes < 64u) lanes <<= 1;
+        }
+        uint32_t pow2 = 1;
+        while (pow2 < lanes && pow2 < 64u) pow2 <<= 1;
+        sd[i].lanes = pow2;
         const jg_hazard_filter& f = p->filters[i];
         sd[i].filter = FilterView{nullptr, 0, 0, 0, 0};
         if (f.kind == JG_FILTER_HASHED) {
@@ -262,7 +321,8 @@ jg_status plan_build_table(jg_plan* p) {
     p->n_units = (uint32_t)units.size();
     const size_t o_units = (sizeof(SetDesc) * sd.size() + 255) & ~(size_t)255;
     const size_t o_ctr = (o_units + sizeof(Unit) * units.size() + 255) & ~(size_t)255;
-    const size_t total = o_ctr + 256;
+    const size_t o_grp = o_ctr + 256;
+    const size_t total = o_grp + sizeof(Group) * (units.size() + 1);
     std::vector<uint8_t> host(total, 0);
     std::memcpy(host.data(), sd.data(), sizeof(SetDesc) * sd.size());
     std::memcpy(host.data() + o_units, units.data(), sizeof(Unit) * units.size());
@@ -272,6 +332,7 @@ jg_status plan_build_table(jg_plan* p) {
     if (st != JG_OK) return st;
     p->off_units = o_units;
     p->off_counter = o_ctr;
+    p->off_groups = o_grp;
     p->generations = gens;
     if (p->gexec) {                 // a captured graph refers to the old table
         (void)hipGraphExecDestroy(p->gexec);
@@ -282,18 +343,22 @@ jg_status plan_build_table(jg_plan* p) {
     int dev = 0, cus = 256;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const uint32_t want = p->cfg.workgroups ? p->cfg.workgroups : (uint32_t)cus * 8u;
-    p->grid = std::max<uint32_t>(1u, std::min<uint32_t>(want, (p->n_units + 3u) / 4u));
+    const uint32_t want = p->cfg.workgroups ? p->cfg.workgroups : (uint32_t)cus * 4u;
+    p->grid_s = std::max<uint32_t>(1u, std::min<uint32_t>(want, (p->n_units + 3u) / 4u));
+    p->grid_p = std::max<uint32_t>(1u, std::min<uint32_t>(want, (p->n_units + 3u) / 4u));
     return JG_OK;
 }
 
 jg_status plan_launch(jg_plan* p, const float* d_m6, uint8_t* d_status, hipStream_t stream) {
     uint8_t* base = (uint8_t*)p->table.ptr;
-    uint32_t* counter = (uint32_t*)(base + p->off_counter);
-    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, stream, counter);
-    hipLaunchKernelGGL(k_detect_batched, dim3(p->grid), dim3(256), 0, stream, (const SetDesc*)base,
-                       (const Unit*)(base + p->off_units), p->n_units, counter, d_m6, d_status);
-    return hip_check(hipGetLastError(), "k_detect_batched launch");
+    uint32_t* counters = (uint32_t*)(base + p->off_counter);
+    Group* groups = (Group*)(base + p->off_groups);
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream, counters);
+    hipLaunchKernelGGL(k_surrogate, dim3(p->grid_s), dim3(256), 0, stream, (const SetDesc*)base,
+                       (const Unit*)(base + p->off_units), p->n_units, counters, groups, d_m6, d_status);
+    hipLaunchKernelGGL(k_poly, dim3(p->grid_p), dim3(256), 0, stream, (const SetDesc*)base, counters,
+                       (const Group*)groups, d_m6, d_status);
+    return hip_check(hipGetLastError(), "batched query launch");
 }
 
 }  // namespace

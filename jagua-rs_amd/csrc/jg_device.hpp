@@ -10,7 +10,14 @@ namespace jg {
 
 constexpr uint32_t ENTIRE_BIT = 0x80000000u;
 constexpr uint32_t SLOT_MASK = 0x7fffffffu;
+constexpr uint32_t COUNT_MASK = 0x3fffffffu;
 constexpr uint32_t HIT = 0xffffffffu;
+
+struct DevNode {                      // mirrors FlatNode (32 B)
+    float4 bbox;
+    int32_t first_child;
+    uint32_t haz_begin, haz_info, n_active_edges;
+};
 
 struct DevEntry {                     // mirrors FlatEntry (32 B)
     float4 ff;
@@ -18,8 +25,7 @@ struct DevEntry {                     // mirrors FlatEntry (32 B)
 };
 
 struct CdeView {
-    const float4* node_bbox;
-    const int4* node_info;            // first_child, haz_begin, haz_count, n_active_edges
+    const DevNode* nodes;
     const uint32_t* node_up;          // parent << 2 | sibling index
     const DevEntry* ent;
     const float4* edges;
@@ -51,6 +57,26 @@ __device__ __forceinline__ bool irrelevant(const FilterView& f, uint64_t q, uint
 __device__ __forceinline__ Mat load_mat(const float* m6, uint64_t i) {
     const float* p = m6 + 6 * i;
     return Mat{p[0], p[1], p[2], p[3], p[4], p[5]};
+}
+
+// QTHazardVec::strongest (qt_hazard_vec.rs:60-63): index of the first relevant entry of
+// node `nd`, or -1.  Without a filter it is the first entry, whose presence the node record
+// carries, so no entry is read.
+__device__ __forceinline__ int strongest(const CdeView& c, const DevNode& nd, const FilterView& f, uint64_t q,
+                                         bool& entire) {
+    const uint32_t cnt = nd.haz_info & COUNT_MASK;
+    if (f.kind == 0) {
+        entire = (nd.haz_info & ENTIRE_BIT) != 0u;
+        return cnt ? (int)nd.haz_begin : -1;
+    }
+    for (uint32_t i = 0; i < cnt; i++) {
+        const uint32_t sf = c.ent[nd.haz_begin + i].slot_flags;
+        if (!irrelevant(f, q, sf & SLOT_MASK)) {
+            entire = (sf & ENTIRE_BIT) != 0u;
+            return (int)(nd.haz_begin + i);
+        }
+    }
+    return -1;
 }
 
 // Depth-first descent in the reference's quadrant order without a memory stack: the
@@ -90,15 +116,16 @@ __device__ __forceinline__ bool descend(const CdeView& c, int start, Visit visit
     }
 }
 
-// CDEngine::get_virtual_root (cd_engine.rs:314-327)
+// CDEngine::get_virtual_root (cd_engine.rs:314-327); children's boxes are the quadrants of
+// the node's box (rect.rs:141-151), exactly as the tree built them.
 __device__ __forceinline__ int virtual_root(const CdeView& c, float4 bb) {
     int n = 0;
     for (;;) {
-        const int fc = c.node_info[n].x;
-        if (fc < 0) return n;
+        const DevNode nd = c.nodes[n];
+        if (nd.first_child < 0) return n;
         int next = -1;
         for (int k = 0; k < 4; k++)
-            if (box_relation(c.node_bbox[fc + k], bb) == REL_SURROUNDING) { next = fc + k; break; }
+            if (box_relation(box_quadrant(nd.bbox, k), bb) == REL_SURROUNDING) { next = nd.first_child + k; break; }
         if (next < 0) return n;
         n = next;
     }
@@ -108,19 +135,18 @@ __device__ __forceinline__ int virtual_root(const CdeView& c, float4 bb) {
 __device__ __forceinline__ bool collides_circle(const CdeView& c, int start, float cx, float cy, float r,
                                                 const FilterView& f, uint64_t q) {
     return descend(c, start, [&](int n, int& fc) -> uint32_t {
-        const int4 info = c.node_info[n];
-        const int he = info.y + info.z;
-        int s = info.y;
-        while (s < he && irrelevant(f, q, c.ent[s].slot_flags & SLOT_MASK)) s++;
-        if (s == he) return 0u;
-        if (c.ent[s].slot_flags & ENTIRE_BIT) return HIT;
-        if (info.x >= 0) {
-            fc = info.x;
-            const float4 nb = c.node_bbox[n];
+        const DevNode nd = c.nodes[n];
+        bool entire = false;
+        const int s = strongest(c, nd, f, q, entire);
+        if (s < 0) return 0u;
+        if (entire) return HIT;
+        if (nd.first_child >= 0) {
+            fc = nd.first_child;
             uint32_t m = 0;
-            for (int k = 0; k < 4; k++) m |= circle_box(cx, cy, r, box_quadrant(nb, k)) ? (1u << k) : 0u;
+            for (int k = 0; k < 4; k++) m |= circle_box(cx, cy, r, box_quadrant(nd.bbox, k)) ? (1u << k) : 0u;
             return m;
         }
+        const int he = (int)(nd.haz_begin + (nd.haz_info & COUNT_MASK));
         for (int i = s; i < he; i++) {
             const DevEntry e = c.ent[i];
             if (irrelevant(f, q, e.slot_flags & SLOT_MASK)) continue;
@@ -135,19 +161,18 @@ __device__ __forceinline__ bool collides_circle(const CdeView& c, int start, flo
 // QTNode::collides for an edge (qt_node.rs:99-147; Edge quadrant test qt_traits.rs:19-106)
 __device__ __forceinline__ bool collides_edge(const CdeView& c, int start, float4 ed, const FilterView& f, uint64_t q) {
     return descend(c, start, [&](int n, int& fc) -> uint32_t {
-        const int4 info = c.node_info[n];
-        const int he = info.y + info.z;
-        int s = info.y;
-        while (s < he && irrelevant(f, q, c.ent[s].slot_flags & SLOT_MASK)) s++;
-        if (s == he) return 0u;
-        if (c.ent[s].slot_flags & ENTIRE_BIT) return HIT;
-        if (info.x >= 0) {
-            fc = info.x;
-            const float4 nb = c.node_bbox[n];
+        const DevNode nd = c.nodes[n];
+        bool entire = false;
+        const int s = strongest(c, nd, f, q, entire);
+        if (s < 0) return 0u;
+        if (entire) return HIT;
+        if (nd.first_child >= 0) {
+            fc = nd.first_child;
             float4 qs[4];
-            for (int k = 0; k < 4; k++) qs[k] = box_quadrant(nb, k);
-            return edge_quadrants(ed, nb, qs);
+            for (int k = 0; k < 4; k++) qs[k] = box_quadrant(nd.bbox, k);
+            return edge_quadrants(ed, nd.bbox, qs);
         }
+        const int he = (int)(nd.haz_begin + (nd.haz_info & COUNT_MASK));
         for (int i = s; i < he; i++) {
             const DevEntry e = c.ent[i];
             if (irrelevant(f, q, e.slot_flags & SLOT_MASK)) continue;

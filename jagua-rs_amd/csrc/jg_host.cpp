@@ -557,23 +557,21 @@ void flatten(const QuadTree& qt, const SlotMap& map, FlatCDE& f) {
             }
     }
     const size_t N = order.size();
-    f.node_bbox.resize(N);
-    f.node_info.resize(N);
+    f.nodes.resize(N);
     f.node_up = up;
     size_t next_child = 1;
     for (size_t i = 0; i < N; i++) {
         const QNode& n = qt.pool[order[i]];
-        int4 info;
-        info.x = -1;
+        FlatNode& fn = f.nodes[i];
+        fn.bbox = n.bbox;
+        fn.first_child = -1;
         if (n.child >= 0) {
-            info.x = (int)next_child;
+            fn.first_child = (int32_t)next_child;
             next_child += 4;
         }
-        info.y = (int)f.ent.size();
-        info.z = (int)n.haz.size();
-        info.w = (int)std::min<uint64_t>(n.n_active_edges, 0x7fffffffu);
-        f.node_bbox[i] = n.bbox;
-        f.node_info[i] = info;
+        fn.haz_begin = (uint32_t)f.ent.size();
+        fn.haz_info = (uint32_t)n.haz.size() | (!n.haz.empty() && n.haz[0].presence == PRES_ENTIRE ? 0x80000000u : 0u);
+        fn.n_active_edges = (uint32_t)std::min<uint64_t>(n.n_active_edges, 0xffffffffu);
         const bool tests_edges = n.child < 0 || n.n_active_edges <= qt.cd_threshold;
         for (const QEntry& h : n.haz) {
             FlatEntry fe{};

@@ -101,9 +101,17 @@ struct FlatEntry {                       // 32 B
     uint32_t n_edges_total;              // partial edge count (n_active_edges bookkeeping)
 };
 
+// One quadtree node, 32 B: everything a descent step reads in one load.
+struct FlatNode {
+    float4 bbox;
+    int32_t first_child;                 // BFS index of the first of 4 children, -1 for a leaf
+    uint32_t haz_begin;                  // first hazard entry
+    uint32_t haz_info;                   // entry count | 0x80000000 when the first entry is Entire
+    uint32_t n_active_edges;             // partial edges over all entries (saturated)
+};
+
 struct FlatCDE {
-    std::vector<float4> node_bbox;
-    std::vector<int4> node_info;         // first_child (BFS, -1 leaf), haz_begin, haz_count, n_active_edges
+    std::vector<FlatNode> nodes;
     std::vector<uint32_t> node_up;       // parent << 2 | index among siblings
     std::vector<FlatEntry> ent;
     std::vector<float4> edges;           // inline hazard edges of entries in leaves / threshold nodes
