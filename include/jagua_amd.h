@@ -233,10 +233,22 @@ jg_status jg_plan_detect(jg_plan* plan, const float* d_mat6, uint8_t* d_status, 
 /* The same launch captured once in a hipGraph and replayed (launch overhead amortised). */
 jg_status jg_plan_detect_graph(jg_plan* plan, const float* d_mat6, uint8_t* d_status, uint32_t replays,
                                void* stream);
+/* Schedule autotuner (no reference counterpart: the reference evaluates one query at a
+ * time).  Times the plan's launch on `stream` with HIP events for candidate schedules —
+ * lanes per query in the polygon stage {per item, 8, 16, 32, 64}, workgroups {1, 2, 4, 8}
+ * per compute unit, queries per work unit {16, 32, 64} — by coordinate descent in that
+ * order (median of `reps` timed launches each, after one warm-up), and keeps the fastest
+ * in the plan.  The plan's flags are kept.  Every schedule computes the same verdicts;
+ * d_status holds them afterwards.  `best` and `best_ms` may be NULL. */
+jg_status jg_plan_autotune(jg_plan* plan, const float* d_mat6, uint8_t* d_status, uint32_t reps, void* stream,
+                           jg_plan_config* best, float* best_ms);
+/* The schedule a plan currently runs with (after defaults are resolved: lanes_per_query 0
+ * means per item). */
+jg_status jg_plan_get_config(const jg_plan* plan, jg_plan_config* out);
 
 /* ---------------------------------------------------------------- overlap gradient */
 
-/* Pole-pair overlap quantity (north-star extension; no reference counterpart; its
+/* Pole-pair overlap quantity (an extension with no reference counterpart; its
  * primitives are Circle::separation_distance for circles and for edges, circle.rs:184-229):
  *   Q = sum over candidate item pairs (a,b), poles i of a, j of b, with |c_i - c_j| < r_i + r_j,
  *         of (sqrt((r_i + r_j)^2) - |c_i - c_j|)^2

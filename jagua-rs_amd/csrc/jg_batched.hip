@@ -54,6 +54,22 @@ struct Group {             // 16 B: surviving queries q0 + i for the set bits i 
 // counters: [0] surrogate units taken, [1] groups appended, [2] groups taken
 constexpr int C_UNITS = 0, C_GROUPS = 1, C_TAKEN = 2;
 
+// per-workgroup LDS of the deferred leaf lists (4 waves)
+constexpr uint32_t DEFER_CAP = 512;
+struct DeferLds {
+    uint32_t items[4][DEFER_CAP];
+    uint32_t count[4];
+    float4 geom[4][64];
+    uint32_t kind[4][64];
+    uint64_t qidx[4][64];
+    uint32_t hit[4][64];
+};
+
+__device__ __forceinline__ Deferred wave_deferred(DeferLds& L) {
+    const uint32_t w = threadIdx.x >> 6;
+    return Deferred{L.items[w], &L.count[w], DEFER_CAP, L.geom[w], L.kind[w], L.qidx[w], L.hit[w]};
+}
+
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
 template <typename T>
@@ -65,11 +81,14 @@ __global__ void k_reset_counters(uint32_t* counters) {
     if (threadIdx.x < 4) counters[threadIdx.x] = 0u;
 }
 
+template <bool FILT, int KIND>
 __global__ __launch_bounds__(256) void k_surrogate(const SetDesc* __restrict__ sets, const Unit* __restrict__ units,
                                                    uint32_t n_units, uint32_t* __restrict__ counters,
                                                    Group* __restrict__ groups, const float* __restrict__ m6,
                                                    uint8_t* __restrict__ status) {
     const uint32_t lane = threadIdx.x & 63u;
+    __shared__ DeferLds lds;
+    const Deferred D = wave_deferred(lds);
     for (;;) {
         uint32_t u = 0;
         if (lane == 0) u = atomicAdd(&counters[C_UNITS], 1u);
@@ -94,19 +113,34 @@ __global__ __launch_bounds__(256) void k_surrogate(const SetDesc* __restrict__ s
             for (uint32_t b = 0; b < cnt; b += qpp) {
                 const uint32_t qi = b + lane / g;
                 bool hit = false;
-                if (qi < cnt) {
-                    const uint64_t q = (uint64_t)un.q0 + qi;
-                    const Mat m = load_mat(m6, sd.first + q);
-                    for (uint32_t e = lane % g; e < n_ent && !hit; e += g) {
+                // rounds of one entity per lane (a single round unless n_ent > 64); leaves are
+                // deferred to a wave-wide test
+                for (uint32_t e0 = 0; e0 < n_ent; e0 += g) {
+                    if (lane == 0) *D.count = 0u;
+                    D.hit[lane] = 0u;
+                    wave_sync_lds();
+                    const uint32_t e = e0 + lane % g;
+                    bool h = false;
+                    if (qi < cnt && e < n_ent) {
+                        const uint64_t q = (uint64_t)un.q0 + qi;
+                        const Mat m = load_mat(m6, sd.first + q);
                         const float4 p = it.ff[e];
-                        if (e < it.n_poles) {
+                        const bool circle = e < it.n_poles;
+                        float4 gm;
+                        if (circle) {
                             const float2 cc = xform(m, p.x, p.y);
-                            hit = collides_circle(c, 0, cc.x, cc.y, p.z, f, q);
+                            gm = make_float4(cc.x, cc.y, p.z, 0.0f);
                         } else {
                             const float2 a = xform(m, p.x, p.y), bb = xform(m, p.z, p.w);
-                            hit = collides_edge(c, 0, make_float4(a.x, a.y, bb.x, bb.y), f, q);
+                            gm = make_float4(a.x, a.y, bb.x, bb.y);
                         }
+                        D.geom[lane] = gm;
+                        D.kind[lane] = circle ? 1u : 0u;
+                        D.qidx[lane] = q;
+                        h = descend_deferring<FILT, KIND>(c, 0, circle, gm, f, q, lane, D);
                     }
+                    h = test_deferred<FILT, KIND>(c, f, D, lane) || h;
+                    hit = hit || h;
                 }
                 const uint64_t bal = ballot(hit);
                 for (uint32_t j = 0; j < qpp && b + j < cnt; j++) {
@@ -128,10 +162,13 @@ __global__ __launch_bounds__(256) void k_surrogate(const SetDesc* __restrict__ s
     }
 }
 
+template <bool FILT>
 __global__ __launch_bounds__(256) void k_poly(const SetDesc* __restrict__ sets, uint32_t* __restrict__ counters,
                                               const Group* __restrict__ groups, const float* __restrict__ m6,
                                               uint8_t* __restrict__ status) {
     const uint32_t lane = threadIdx.x & 63u;
+    __shared__ DeferLds lds;
+    const Deferred D = wave_deferred(lds);
     const uint32_t n_groups = __atomic_load_n(&counters[C_GROUPS], __ATOMIC_RELAXED);
     for (;;) {
         uint32_t gi = 0;
@@ -183,12 +220,20 @@ __global__ __launch_bounds__(256) void k_poly(const SetDesc* __restrict__ sets, 
                 if (ballot(!done) == 0ull) break;
                 const uint32_t i = i0 + sub;
                 bool hit = false;
+                if (lane == 0) *D.count = 0u;
+                D.hit[lane] = 0u;
+                wave_sync_lds();
                 if (!done && i < it.n_v) {
                     const uint32_t j = i + 1 == it.n_v ? 0 : i + 1;
                     const float2 a = xform(m, it.v[i].x, it.v[i].y);
                     const float2 b2 = xform(m, it.v[j].x, it.v[j].y);
-                    hit = collides_edge(c, vr, make_float4(a.x, a.y, b2.x, b2.y), f, q);
+                    const float4 ed = make_float4(a.x, a.y, b2.x, b2.y);
+                    D.geom[lane] = ed;
+                    D.kind[lane] = 0u;
+                    D.qidx[lane] = q;
+                    hit = descend_deferring<FILT, KIND_EDGE>(c, vr, false, ed, f, q, lane, D);
                 }
+                hit = test_deferred<FILT, KIND_EDGE>(c, f, D, lane) || hit;
                 const uint64_t bal = ballot(hit);
                 if (!done && ((bal >> (slot * G)) & gmask)) {
                     res = JG_Q_COLLISION;
@@ -207,7 +252,7 @@ __global__ __launch_bounds__(256) void k_poly(const SetDesc* __restrict__ sets, 
                     const float2 qpoi = xform(m, it.poi.x, it.poi.y);
                     for (uint32_t i = sub; i < cntv; i += G) {
                         const uint32_t slot_h = c.ent[vn.haz_begin + i].slot_flags & SLOT_MASK;
-                        if (irrelevant(f, q, slot_h)) continue;
+                        if (irrel<FILT>(f, q, slot_h)) continue;
                         if (containment(c, slot_h, it, m, bb, qpoi)) { hit = true; break; }
                     }
                 }
@@ -231,6 +276,9 @@ struct jg_plan {
     uint32_t n_units = 0;
     size_t off_units = 0, off_counter = 0, off_groups = 0;
     uint32_t grid_s = 0, grid_p = 0;
+    uint32_t n_cus = 0;
+    bool filtered = false;          // some set has a filter (kernels with the filter test)
+    int kind = KIND_CIRCLE;         // entity kinds of the surrogate stage over all sets
     // graph cache
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -256,6 +304,10 @@ jg_status plan_build_table(jg_plan* p) {
     std::vector<uint64_t> gens;
     for (auto& s : p->sets) gens.push_back(s.cde->generation);
     if (gens == p->generations && p->table.ptr) return JG_OK;
+    if (p->table.ptr) {             // launches in flight on any stream may still read the table
+        jg_status e = hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        if (e != JG_OK) return e;
+    }
 
     // filter masks
     std::vector<uint32_t> mask_words;
@@ -278,6 +330,7 @@ jg_status plan_build_table(jg_plan* p) {
         if (e != JG_OK) return e;
     }
     std::vector<SetDesc> sd(p->sets.size());
+    bool any_filter = false, any_pole = false, any_pier = false;
     for (size_t i = 0; i < p->sets.size(); i++) {
         const jg_query_set& s = p->sets[i];
         sd[i].cde = s.cde->view;
@@ -304,7 +357,12 @@ jg_status plan_build_table(jg_plan* p) {
             sd[i].filter.mask = (const uint32_t*)p->masks.ptr + mask_off[i];
             sd[i].filter.n_words = (uint32_t)((s.cde->map.slots.size() + 31) / 32);
         }
+        any_filter = any_filter || sd[i].filter.kind != 0;
+        any_pole = any_pole || sd[i].item.n_poles > 0;
+        any_pier = any_pier || sd[i].item.n_piers > 0;
     }
+    p->filtered = any_filter;
+    p->kind = any_pier ? (any_pole ? KIND_MIXED : KIND_EDGE) : KIND_CIRCLE;
     // work units: sets in decreasing estimated cost, chunks of `chunk` queries
     const uint32_t chunk = p->cfg.chunk ? std::min<uint32_t>(p->cfg.chunk, 64u) : 64u;
     std::vector<size_t> order(p->sets.size());
@@ -343,6 +401,7 @@ jg_status plan_build_table(jg_plan* p) {
     int dev = 0, cus = 256;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    p->n_cus = (uint32_t)cus;
     const uint32_t want = p->cfg.workgroups ? p->cfg.workgroups : (uint32_t)cus * 4u;
     p->grid_s = std::max<uint32_t>(1u, std::min<uint32_t>(want, (p->n_units + 3u) / 4u));
     p->grid_p = std::max<uint32_t>(1u, std::min<uint32_t>(want, (p->n_units + 3u) / 4u));
@@ -354,10 +413,23 @@ jg_status plan_launch(jg_plan* p, const float* d_m6, uint8_t* d_status, hipStrea
     uint32_t* counters = (uint32_t*)(base + p->off_counter);
     Group* groups = (Group*)(base + p->off_groups);
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream, counters);
-    hipLaunchKernelGGL(k_surrogate, dim3(p->grid_s), dim3(256), 0, stream, (const SetDesc*)base,
-                       (const Unit*)(base + p->off_units), p->n_units, counters, groups, d_m6, d_status);
-    hipLaunchKernelGGL(k_poly, dim3(p->grid_p), dim3(256), 0, stream, (const SetDesc*)base, counters,
-                       (const Group*)groups, d_m6, d_status);
+    const SetDesc* sets = (const SetDesc*)base;
+    const Unit* units = (const Unit*)(base + p->off_units);
+    const dim3 gs(p->grid_s), gp(p->grid_p), blk(256);
+#define JG_SURR(F, K) hipLaunchKernelGGL((k_surrogate<F, K>), gs, blk, 0, stream, sets, units, p->n_units, counters, \
+                                         groups, d_m6, d_status)
+    if (p->filtered) {
+        if (p->kind == KIND_CIRCLE) JG_SURR(true, KIND_CIRCLE);
+        else if (p->kind == KIND_EDGE) JG_SURR(true, KIND_EDGE);
+        else JG_SURR(true, KIND_MIXED);
+        hipLaunchKernelGGL(k_poly<true>, gp, blk, 0, stream, sets, counters, (const Group*)groups, d_m6, d_status);
+    } else {
+        if (p->kind == KIND_CIRCLE) JG_SURR(false, KIND_CIRCLE);
+        else if (p->kind == KIND_EDGE) JG_SURR(false, KIND_EDGE);
+        else JG_SURR(false, KIND_MIXED);
+        hipLaunchKernelGGL(k_poly<false>, gp, blk, 0, stream, sets, counters, (const Group*)groups, d_m6, d_status);
+    }
+#undef JG_SURR
     return hip_check(hipGetLastError(), "batched query launch");
 }
 
@@ -444,5 +516,91 @@ extern "C" jg_status jg_plan_detect_graph(jg_plan* p, const float* d_m6, uint8_t
         st = hip_check(hipGraphLaunch(p->gexec, s), "hipGraphLaunch");
         if (st != JG_OK) return st;
     }
+    return JG_OK;
+}
+
+extern "C" jg_status jg_plan_get_config(const jg_plan* p, jg_plan_config* out) {
+    if (!p || !out) return fail(JG_ERR_INVALID_ARG, "null argument");
+    *out = p->cfg;
+    return JG_OK;
+}
+
+namespace {
+
+// median of `reps` timed launches of the plan with its current schedule (after one warm-up)
+jg_status plan_time(jg_plan* p, const float* d_m6, uint8_t* d_status, uint32_t reps, hipStream_t s,
+                    hipEvent_t a, hipEvent_t b, float& ms) {
+    p->generations.clear();                 // rebuild the table with the current schedule
+    jg_status st = plan_build_table(p);
+    if (st != JG_OK) return st;
+    st = plan_launch(p, d_m6, d_status, s);
+    if (st != JG_OK) return st;
+    std::vector<float> t;
+    for (uint32_t r = 0; r < reps; r++) {
+        st = hip_check(hipEventRecord(a, s), "hipEventRecord");
+        if (st == JG_OK) st = plan_launch(p, d_m6, d_status, s);
+        if (st == JG_OK) st = hip_check(hipEventRecord(b, s), "hipEventRecord");
+        if (st == JG_OK) st = hip_check(hipEventSynchronize(b), "hipEventSynchronize");
+        float x = 0.0f;
+        if (st == JG_OK) st = hip_check(hipEventElapsedTime(&x, a, b), "hipEventElapsedTime");
+        if (st != JG_OK) return st;
+        t.push_back(x);
+    }
+    std::sort(t.begin(), t.end());
+    ms = t[t.size() / 2];
+    return JG_OK;
+}
+
+}  // namespace
+
+extern "C" jg_status jg_plan_autotune(jg_plan* p, const float* d_m6, uint8_t* d_status, uint32_t reps, void* stream,
+                                      jg_plan_config* best, float* best_ms) {
+    if (!p) return fail(JG_ERR_INVALID_ARG, "null plan");
+    if (!d_m6 || !d_status) return fail(JG_ERR_INVALID_ARG, "null device buffer");
+    if (reps == 0) reps = 3;
+    if (p->n_units == 0) {
+        if (best) *best = p->cfg;
+        if (best_ms) *best_ms = 0.0f;
+        return JG_OK;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t a = nullptr, b = nullptr;
+    jg_status st = hip_check(hipEventCreate(&a), "hipEventCreate");
+    if (st == JG_OK) st = hip_check(hipEventCreate(&b), "hipEventCreate");
+    jg_plan_config cur = p->cfg;
+    float cur_ms = 0.0f;
+    if (st == JG_OK) st = plan_time(p, d_m6, d_status, reps, s, a, b, cur_ms);
+    const uint32_t cus = p->n_cus ? p->n_cus : 256u;
+    const std::vector<uint32_t> lanes = {0u, 8u, 16u, 32u, 64u};
+    const std::vector<uint32_t> wgs = {cus, 2u * cus, 4u * cus, 8u * cus};
+    const std::vector<uint32_t> chunks = {16u, 32u, 64u};
+    for (int dim = 0; dim < 3 && st == JG_OK; dim++) {
+        const std::vector<uint32_t>& vals = dim == 0 ? lanes : dim == 1 ? wgs : chunks;
+        for (uint32_t v : vals) {
+            jg_plan_config c = cur;
+            uint32_t& field = dim == 0 ? c.lanes_per_query : dim == 1 ? c.workgroups : c.chunk;
+            const uint32_t was = dim == 0 ? cur.lanes_per_query : dim == 1 ? cur.workgroups : cur.chunk;
+            if (v == was) continue;
+            field = v;
+            p->cfg = c;
+            float ms = 0.0f;
+            st = plan_time(p, d_m6, d_status, reps, s, a, b, ms);
+            if (st != JG_OK) break;
+            if (ms < cur_ms) {
+                cur = c;
+                cur_ms = ms;
+            }
+        }
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    p->cfg = cur;
+    p->generations.clear();
+    jg_status st2 = plan_build_table(p);
+    if (st == JG_OK) st = st2;
+    if (st == JG_OK) st = plan_launch(p, d_m6, d_status, s);   // d_status from the kept schedule
+    if (st != JG_OK) return st;
+    if (best) *best = cur;
+    if (best_ms) *best_ms = cur_ms;
     return JG_OK;
 }

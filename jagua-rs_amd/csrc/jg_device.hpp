@@ -184,6 +184,124 @@ __device__ __forceinline__ bool collides_edge(const CdeView& c, int start, float
     });
 }
 
+// ---------------------------------------------------------------------------------------
+// Deferred leaf tests.  A wave's lanes descend independently; a lane that reaches a leaf
+// whose strongest relevant hazard is Partial records (lane, leaf) in a per-wave LDS list
+// instead of testing it inline, and the wave then tests all recorded leaves with all of its
+// lanes.  The verdict of a lane is the OR of its Entire hits and its leaf hits, exactly the
+// result of QTNode::collides (qt_node.rs:99-147), whose recursion only decides which hazard
+// is reported first.
+
+struct Deferred {                     // per-wave LDS
+    uint32_t* items;                  // lane << 26 | leaf node
+    uint32_t* count;
+    uint32_t cap;
+    float4* geom;                     // per lane: circle (cx, cy, r, 1) or edge (x1 y1 x2 y2)
+    uint32_t* kind;                   // per lane: 1 circle, 0 edge
+    uint64_t* qidx;                   // per lane: query index (hashed filters)
+    uint32_t* hit;                    // per lane: leaf hit flag
+};
+
+// Entity kinds of a descent: KIND_EDGE and KIND_CIRCLE are fixed at compile time;
+// KIND_MIXED reads the per-lane kind (surrogates with both poles and piers).
+enum : int { KIND_EDGE = 0, KIND_CIRCLE = 1, KIND_MIXED = 2 };
+
+template <bool FILT>
+__device__ __forceinline__ bool irrel(const FilterView& f, uint64_t q, uint32_t slot) {
+    return FILT && irrelevant(f, q, slot);
+}
+
+template <bool FILT>
+__device__ __forceinline__ int strongest_t(const CdeView& c, const DevNode& nd, const FilterView& f, uint64_t q,
+                                           bool& entire) {
+    if (!FILT) {
+        entire = (nd.haz_info & ENTIRE_BIT) != 0u;
+        return (nd.haz_info & COUNT_MASK) ? (int)nd.haz_begin : -1;
+    }
+    return strongest(c, nd, f, q, entire);
+}
+
+// QTHazPartial::collides_with over the relevant Partial entries of a leaf, from entry s on
+// (qt_node.rs:126-141, qt_partial_hazard.rs:67-72).
+template <bool FILT, int KIND>
+__device__ __forceinline__ bool leaf_hits(const CdeView& c, const DevNode& nd, int s, bool circle_rt, float4 g,
+                                          const FilterView& f, uint64_t q) {
+    const bool circle = KIND == KIND_CIRCLE || (KIND == KIND_MIXED && circle_rt);
+    const int he = (int)(nd.haz_begin + (nd.haz_info & COUNT_MASK));
+    for (int i = s; i < he; i++) {
+        const DevEntry e = c.ent[i];
+        if (irrel<FILT>(f, q, e.slot_flags & SLOT_MASK)) continue;
+        if (circle) {
+            if (!circle_box(g.x, g.y, g.z, e.ff)) continue;
+            for (uint32_t j = 0; j < e.edge_count; j++)
+                if (circle_edge(g.x, g.y, g.z, c.edges[e.edge_begin + j])) return true;
+        } else {
+            if (!box_edge(e.ff, g)) continue;
+            for (uint32_t j = 0; j < e.edge_count; j++)
+                if (edge_edge(g, c.edges[e.edge_begin + j])) return true;
+        }
+    }
+    return false;
+}
+
+// Descent of one entity (circle: g = (cx, cy, r, -); edge: g = segment) from `start`,
+// returning true on an Entire hit; leaves are deferred (tested inline when the list is full).
+template <bool FILT, int KIND>
+__device__ __forceinline__ bool descend_deferring(const CdeView& c, int start, bool circle_rt, float4 g,
+                                                  const FilterView& f, uint64_t q, uint32_t lane, const Deferred& D) {
+    const bool circle = KIND == KIND_CIRCLE || (KIND == KIND_MIXED && circle_rt);
+    return descend(c, start, [&](int n, int& fc) -> uint32_t {
+        const DevNode nd = c.nodes[n];
+        bool entire = false;
+        const int s = strongest_t<FILT>(c, nd, f, q, entire);
+        if (s < 0) return 0u;
+        if (entire) return HIT;
+        if (nd.first_child >= 0) {
+            fc = nd.first_child;
+            if (circle) {
+                uint32_t m = 0;
+                for (int k = 0; k < 4; k++) m |= circle_box(g.x, g.y, g.z, box_quadrant(nd.bbox, k)) ? (1u << k) : 0u;
+                return m;
+            }
+            float4 qs[4];
+            for (int k = 0; k < 4; k++) qs[k] = box_quadrant(nd.bbox, k);
+            return edge_quadrants(g, nd.bbox, qs);
+        }
+        const uint32_t at = atomicAdd(D.count, 1u);
+        if (at < D.cap) {
+            D.items[at] = (lane << 26) | (uint32_t)n;
+            return 0u;
+        }
+        return leaf_hits<FILT, KIND>(c, nd, s, circle_rt, g, f, q) ? HIT : 0u;
+    });
+}
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Tests every deferred leaf with the wave's lanes (round-robin) and returns this lane's flag.
+template <bool FILT, int KIND>
+__device__ __forceinline__ bool test_deferred(const CdeView& c, const FilterView& f, const Deferred& D, uint32_t lane) {
+    wave_sync_lds();
+    const uint32_t n = min(*D.count, D.cap);
+    for (uint32_t i = lane; i < n; i += 64) {
+        const uint32_t it = D.items[i];
+        const uint32_t owner = it >> 26;
+        if (D.hit[owner]) continue;
+        const DevNode nd = c.nodes[it & 0x3ffffffu];
+        const uint64_t q = FILT ? D.qidx[owner] : 0ull;
+        bool entire = false;
+        const int s = strongest_t<FILT>(c, nd, f, q, entire);
+        if (s >= 0 && leaf_hits<FILT, KIND>(c, nd, s, KIND == KIND_MIXED && D.kind[owner] != 0u, D.geom[owner], f, q))
+            D.hit[owner] = 1u;
+    }
+    wave_sync_lds();
+    return D.hit[lane] != 0u;
+}
+
 // Hazard-polygon point-in-polygon (simple_polygon.rs:255-292) over the hazard's full shape.
 __device__ __forceinline__ bool pip_hazard(const CdeView& c, uint32_t slot, float px, float py) {
     const float4 bb = c.haz_bbox[slot];

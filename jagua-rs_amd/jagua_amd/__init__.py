@@ -451,6 +451,25 @@ class QueryPlan:
                                            replays, _stream_ptr(stream)))
         return out
 
+    def autotune(self, mats, out, reps=3, stream=None):
+        """Times candidate schedules on these inputs and keeps the fastest (jg_plan_autotune).
+        Returns (schedule dict, its median ms); `out` holds the verdicts afterwards."""
+        best = N.PlanConfigC()
+        ms = C.c_float()
+        check(N.lib().jg_plan_autotune(self._h, C.c_void_p(mats.data_ptr()), C.c_void_p(out.data_ptr()), reps,
+                                       _stream_ptr(stream), C.byref(best), C.byref(ms)))
+        return self._cfg_dict(best), ms.value
+
+    def config(self):
+        c = N.PlanConfigC()
+        check(N.lib().jg_plan_get_config(self._h, C.byref(c)))
+        return self._cfg_dict(c)
+
+    @staticmethod
+    def _cfg_dict(c):
+        return {"chunk": c.chunk, "workgroups": c.workgroups, "lanes_per_query": c.lanes_per_query,
+                "flags": c.flags}
+
 
 class PoleOverlap:
     """The pole-pair overlap quantity Q (include/jagua_amd.h, jg_overlap_*) as a forward /

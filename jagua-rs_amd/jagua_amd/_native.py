@@ -93,6 +93,9 @@ _SIGS = {
     "jg_plan_free": (None, [c_vp]),
     "jg_plan_detect": (C.c_int32, [c_vp, c_vp, c_vp, c_vp]),
     "jg_plan_detect_graph": (C.c_int32, [c_vp, c_vp, c_vp, C.c_uint32, c_vp]),
+    "jg_plan_autotune": (C.c_int32, [c_vp, c_vp, c_vp, C.c_uint32, c_vp, C.POINTER(PlanConfigC),
+                                     C.POINTER(C.c_float)]),
+    "jg_plan_get_config": (C.c_int32, [c_vp, C.POINTER(PlanConfigC)]),
     "jg_overlap_create": (C.c_int32, [c_f32p, c_u32p, C.c_uint32, c_u32p, C.c_uint64, c_f32p, C.c_uint32,
                                       C.POINTER(c_vp)]),
     "jg_overlap_set_pairs": (C.c_int32, [c_vp, c_u32p, C.c_uint64]),

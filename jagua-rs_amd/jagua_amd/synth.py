@@ -191,3 +191,62 @@ def config4_workload(seed=4, n_items=4096, width=2000.0, height=800.0, r_lo=6.0,
     cedges = np.array([[0, 0, width, 0], [width, 0, width, height], [width, height, 0, height],
                        [0, height, 0, 0]], dtype=F32)
     return dict(poles=allp, off=off, pose=pose, rad=rad, pairs=pairs, cedges=cedges)
+
+
+def _log_uniform_int(rng, lo, hi):
+    return int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+
+
+def config3_layout(seed=3, layout=0):
+    """One of the many independent bin-packing layouts: a star-shaped container of 64-2048
+    vertices (radii U[0.7R, R], R = 40 sqrt(n_items)) holding n_items ~ log-uniform [50, 400]
+    star or concave polygons of 8-256 vertices (radius 10-30); quadtree depth 7."""
+    rng = np.random.default_rng([seed, layout])
+    n_items = _log_uniform_int(rng, 50, 400)
+    big_r = 40.0 * float(np.sqrt(n_items))
+    return generate_scene(depth=7, seed=(seed << 20) + layout, container_kind=1, cw=0.7 * big_r, ch=big_r,
+                          container_nv_lo=64, container_nv_hi=2048, n_items=n_items, item_nv_lo=8,
+                          item_nv_hi=256, item_r_lo=10.0, item_r_hi=30.0, item_shape=2, min_sep=0.0, rot_kind=0,
+                          placement=0, max_attempts=4000)
+
+
+def config3_queries(scene, seed=3, layout=0, n_items=16, n_per_item=4096):
+    """n_items query polygons (8-256 vertices, radius 10-30) per layout, each at n_per_item
+    continuous-rotation transforms over the container's bbox."""
+    c = scene.container
+    box = (float(c[:, 0].min()), float(c[:, 1].min()), float(c[:, 0].max()), float(c[:, 1].max()))
+    base = ((seed << 20) + layout) * 1009
+    polys = [polygon(base + 2 * i, 8, 256, 10.0, 30.0, 2) for i in range(n_items)]
+    dts = [transforms(base + 2 * i + 1, n_per_item, box, 0) for i in range(n_items)]
+    return polys, dts
+
+
+def config3_workload(seed=3, n_layouts=256, n_items=16, n_per_item=4096):
+    """Many layouts in one launch: n_layouts config3 layouts (generated in parallel), their
+    query polygons and transforms, and a HashedFilter per layout that deems a random 10% of
+    the hazards irrelevant per query.  Returns (scenes, [(polys, dts)], filters)."""
+    from . import HashedFilter
+    with cf.ThreadPoolExecutor(_threads()) as ex:
+        scenes = list(ex.map(lambda l: config3_layout(seed, l), range(n_layouts)))
+    queries = [config3_queries(sc, seed, l, n_items, n_per_item) for l, sc in enumerate(scenes)]
+    filters = [HashedFilter((seed << 16) + l, 0.1) for l in range(n_layouts)]
+    return scenes, queries, filters
+
+
+def config5_scene(seed=5, n_items=20000, item_r_lo=6.0, item_r_hi=12.0, width=6000.0, height=2000.0, n_holes=20):
+    """Full scale: a 6000x2000 rectangle with 20 star-shaped holes of 8-64 vertices (radius
+    60-200) and n_items concave polygons (3-octave value-noise radii, amplitude 0.3) of
+    log-uniform 64-1024 vertices; quadtree depth 10."""
+    return generate_scene(depth=10, seed=seed, container_kind=0, cw=width, ch=height, n_holes=n_holes, hole_kind=1,
+                          hole_lo=60.0, hole_hi=200.0, hole_nv_lo=8, hole_nv_hi=64, n_items=n_items, item_nv_lo=64,
+                          item_nv_hi=1024, item_r_lo=item_r_lo, item_r_hi=item_r_hi, item_shape=1, min_sep=0.0,
+                          rot_kind=0, placement=0, max_attempts=2000)
+
+
+def config5_queries(seed=5, n_items=256, n_per_item=65536, r_lo=6.0, r_hi=12.0, width=6000.0, height=2000.0,
+                    nv_hi=1024):
+    """n_items query polygons of mixed shape and size (8-1024 vertices) at n_per_item
+    continuous-rotation transforms over the 6000x2000 container."""
+    polys = [polygon(seed * 100003 + i, 8, nv_hi, r_lo, r_hi, 2) for i in range(n_items)]
+    dts = [transforms(seed * 100003 + 7 * i + 1, n_per_item, (0.0, 0.0, width, height), 0) for i in range(n_items)]
+    return polys, dts

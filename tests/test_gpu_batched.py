@@ -91,3 +91,35 @@ def test_batched_poly_only_and_surrogate_only(cuda):
         np.testing.assert_array_equal(out.cpu().numpy(), plain)
         if flags == J.QUERY_SURROGATE:
             np.testing.assert_array_equal(plain, ref["surr"])
+
+
+def test_autotune_keeps_verdicts(cuda):
+    sc = S.config2_scene(seed=12, n_items=300)
+    cde, keys, oc = engines(sc)
+    polys = [S.polygon(300 + i, 8, 300, 10.0, 30.0, 2) for i in range(4)]
+    prod, orc = items(polys)
+    box = sc.cde_bbox()
+    sets, expect, mats_all, first = [], [], [], 0
+    for i in range(len(polys)):
+        mats = J.compose(S.transforms(500 + i, 3000 + 111 * i, box, 0))
+        filt = J.HashedFilter(9, 0.1) if i % 2 else None
+        of = O.make_filter(2, filt.seed, filt.threshold) if filt else None
+        sets.append((cde, prod[i], first, len(mats), filt))
+        expect.append(O.query(oc, orc[i], mats=mats, mode=1, filt=of, threads=8)["status"])
+        mats_all.append(mats)
+        first += len(mats)
+    m = torch.from_numpy(np.concatenate(mats_all)).to(cuda)
+    out = torch.full((first,), 255, dtype=torch.uint8, device=cuda)
+    plan = J.QueryPlan(sets, flags=0)
+    cfg, ms = plan.autotune(m, out, reps=2)
+    torch.cuda.synchronize()
+    assert ms > 0 and cfg == plan.config()
+    assert cfg["lanes_per_query"] in (0, 8, 16, 32, 64) and 1 <= cfg["chunk"] <= 64 or cfg["chunk"] == 0
+    np.testing.assert_array_equal(out.cpu().numpy(), np.concatenate(expect))
+    out.fill_(255)
+    plan.detect(m, out)
+    np.testing.assert_array_equal(out.cpu().numpy(), np.concatenate(expect))
+    out.fill_(255)
+    plan.detect_graph(m, out, replays=2)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out.cpu().numpy(), np.concatenate(expect))

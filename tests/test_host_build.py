@@ -167,3 +167,18 @@ def test_flattened_tree_records():
             assert (nodes[child:child + 4, 0] == n[0] - 1).all()
             child += 4
     assert child == len(nodes)
+
+
+def test_workload_generators_deterministic():
+    a = S.config3_layout(3, 1)
+    b = S.config3_layout(3, 1)
+    assert 64 <= len(a.container) <= 2048 and 50 <= len(a.items) <= 400
+    assert np.array_equal(a.container, b.container) and len(a.items) == len(b.items)
+    assert all(np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1]) for x, y in zip(a.items, b.items))
+    assert all(8 <= len(v) <= 256 for v, _ in a.items)
+    polys, dts = S.config3_queries(a, 3, 1, n_items=2, n_per_item=16)
+    c = a.container
+    assert np.all((dts[0][:, 1] >= c[:, 0].min()) & (dts[0][:, 1] <= c[:, 0].max()))
+    sc = S.config5_scene(seed=6, n_items=60, width=600.0, height=200.0, n_holes=2)
+    assert sc.depth == 10 and all(64 <= len(v) <= 1024 for v, _ in sc.items)
+    assert all(8 <= len(h) <= 64 for h in sc.holes)
