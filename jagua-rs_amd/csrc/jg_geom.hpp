@@ -198,6 +198,66 @@ JG_HD int pip_edge(float px, float py, float ray_x, float sx, float sy, float ex
     return edge_edge(make_float4(px, py, ray_x, py), make_float4(sx, sy, ex, ey)) ? 1 : 0;
 }
 
+// SPolygon::collides_with(&Point) ray cast (simple_polygon.rs:255-292) over vertices v[0..n)
+// with bounding box bb.
+JG_HD bool poly_has_point(const float2* v, uint32_t n, float4 bb, float x, float y) {
+    if (!box_has_point(bb, x, y)) return false;
+    const float ray_x = bb.z + (bb.z - bb.x);
+    int cnt = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const float2 s = v[i], e = v[i + 1 == n ? 0 : i + 1];
+        cnt += pip_edge(x, y, ray_x, s.x, s.y, e.x, e.y);
+    }
+    return (cnt & 1) == 1;
+}
+
+// Edge::distance_to(Point) (edge.rs:146-159, closest_point_on_edge :63-87)
+JG_HD float edge_point_dist(float4 e, float px, float py) {
+    const float a = px - e.x, b = py - e.y, c = e.z - e.x, d = e.w - e.y;
+    const float dot = a * c + b * d;
+    const float len_sq = c * c + d * d;
+    float param = -1.0f;
+    if (len_sq != 0.0f) param = dot / len_sq;
+    float xx, yy;
+    if (param < 0.0f) { xx = e.x; yy = e.y; }
+    else if (param > 1.0f) { xx = e.z; yy = e.w; }
+    else { xx = e.x + param * c; yy = e.y + param * d; }
+    const float dx = px - xx, dy = py - yy;
+    return sqrtf(dx * dx + dy * dy);
+}
+
+// MAX_POI_TREE_DEPTH (fail_fast/pole.rs:82): levels below the root cell of the pole search
+constexpr int POLE_LEVELS = 10;
+
+// POINode::new (fail_fast/pole.rs:93-127): signed distance of the cell centre to the border
+// of `shape` minus the existing poles (negative when the centre is outside); radius is half
+// the cell's diameter.  Poles are (cx, cy, r, -).
+JG_HD float pole_cell(const float2* v, uint32_t n, float4 bb, const float4* poles, uint32_t n_poles, float4 box,
+                      float& radius) {
+    const float dx = box.z - box.x, dy = box.w - box.y;
+    radius = sqrtf(dx * dx + dy * dy) / 2.0f;
+    const float cx = (box.x + box.z) / 2.0f, cy = (box.y + box.w) / 2.0f;
+    bool inside = poly_has_point(v, n, bb, cx, cy);
+    if (inside)
+        for (uint32_t i = 0; i < n_poles; i++) {
+            const float4 c = poles[i];
+            const float ex = cx - c.x, ey = cy - c.y;
+            if (ex * ex + ey * ey <= c.z * c.z) { inside = false; break; }     // circle.rs:95-107
+        }
+    float d = 3.40282347e+38f;
+    for (uint32_t i = 0; i < n; i++) {
+        const float2 a = v[i], b = v[i + 1 == n ? 0 : i + 1];
+        d = mn(d, edge_point_dist(make_float4(a.x, a.y, b.x, b.y), cx, cy));
+    }
+    for (uint32_t i = 0; i < n_poles; i++) {
+        const float4 c = poles[i];
+        const float ex = cx - c.x, ey = cy - c.y;
+        const float dc = sqrtf(ex * ex + ey * ey);
+        d = mn(d, dc <= c.z ? c.z - dc : dc - c.z);                               // circle.rs:154-163
+    }
+    return inside ? d : -d;
+}
+
 // Hashed per-query hazard filter and set hash (specified in include/jagua_amd.h).
 JG_HD uint32_t fmix32(uint32_t h) {
     h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;

@@ -45,15 +45,7 @@ static bool bbox_of(const std::vector<float2>& v, float4& out) {   // simple_pol
 }
 
 bool polygon_has_point(const Polygon& p, float x, float y) {        // simple_polygon.rs:255-292
-    if (!box_has_point(p.bbox, x, y)) return false;
-    const float ray_x = p.bbox.z + (p.bbox.z - p.bbox.x);
-    int cnt = 0;
-    const uint32_t n = p.n();
-    for (uint32_t i = 0; i < n; i++) {
-        const float2 s = p.v[i], e = p.v[i + 1 == n ? 0 : i + 1];
-        cnt += pip_edge(x, y, ray_x, s.x, s.y, e.x, e.y);
-    }
-    return (cnt & 1) == 1;
+    return poly_has_point(p.v.data(), p.n(), p.bbox, x, y);
 }
 
 // Monotone-chain hull (convex_hull.rs:31-64): stable x-sort, lower then upper chain.
@@ -80,44 +72,17 @@ static float sq_dist(float2 a, float2 b) {                          // point.rs:
     return dx * dx + dy * dy;
 }
 
-// Edge::distance_to(Point) (edge.rs:146-159)
-static float edge_point_dist(float4 e, float px, float py) {
-    const float a = px - e.x, b = py - e.y, c = e.z - e.x, d = e.w - e.y;
-    const float dot = a * c + b * d;
-    const float len_sq = c * c + d * d;
-    float param = -1.0f;
-    if (len_sq != 0.0f) param = dot / len_sq;
-    float xx, yy;
-    if (param < 0.0f) { xx = e.x; yy = e.y; }
-    else if (param > 1.0f) { xx = e.z; yy = e.w; }
-    else { xx = e.x + param * c; yy = e.y + param * d; }
-    const float dx = px - xx, dy = py - yy;
-    return sqrtf(dx * dx + dy * dy);
-}
-
 // Pole of inaccessibility search (fail_fast/pole.rs:52-135): breadth-first over a quadtree of
 // the square bbox, 10 levels, pruned by radius + distance.
 static bool compute_pole(const Polygon& shape, const std::vector<float4>& poles, float4& out) {
     struct Cell { float4 box; int level; float radius, dist; };
     auto make_cell = [&](float4 box, int level) {
-        const float dx = box.z - box.x, dy = box.w - box.y;
-        const float radius = sqrtf(dx * dx + dy * dy) / 2.0f;
-        const float cx = (box.x + box.z) / 2.0f, cy = (box.y + box.w) / 2.0f;
-        bool inside = polygon_has_point(shape, cx, cy);
-        if (inside)
-            for (const float4& c : poles)
-                if (sq_dist(make_float2(cx, cy), make_float2(c.x, c.y)) <= c.z * c.z) { inside = false; break; }
-        float d = FLT_MAX;
-        for (uint32_t i = 0; i < shape.n(); i++) d = mn(d, edge_point_dist(shape.edge(i), cx, cy));
-        for (const float4& c : poles) {
-            const float ddx = cx - c.x, ddy = cy - c.y;
-            const float dc = sqrtf(ddx * ddx + ddy * ddy);
-            d = mn(d, dc <= c.z ? c.z - dc : dc - c.z);             // circle.rs:154-163
-        }
-        return Cell{box, level, radius, inside ? d : -d};
+        Cell c{box, level, 0.0f, 0.0f};
+        c.dist = pole_cell(shape.v.data(), shape.n(), shape.bbox, poles.data(), (uint32_t)poles.size(), box, c.radius);
+        return c;
     };
     std::vector<Cell> fifo;
-    fifo.push_back(make_cell(box_inflate_to_square(shape.bbox), 10));
+    fifo.push_back(make_cell(box_inflate_to_square(shape.bbox), POLE_LEVELS));
     bool have = false;
     float4 best{};
     for (size_t head = 0; head < fifo.size(); head++) {
@@ -139,7 +104,8 @@ static bool compute_pole(const Polygon& shape, const std::vector<float4>& poles,
     return have;
 }
 
-bool polygon_new(const float* xy, uint32_t n, Polygon& out, std::string& err) {   // simple_polygon.rs:41-71
+// SPolygon::new (simple_polygon.rs:41-71) up to, not including, the pole of inaccessibility
+bool polygon_prepare(const float* xy, uint32_t n, Polygon& out, std::string& err) {
     if (n < 3) { err = "simple polygon must have at least 3 points"; return false; }
     std::vector<float2> v(n);
     for (uint32_t i = 0; i < n; i++) v[i] = make_float2(xy[2 * i], xy[2 * i + 1]);
@@ -166,11 +132,17 @@ bool polygon_new(const float* xy, uint32_t n, Polygon& out, std::string& err) { 
     p.diameter = sqrtf(best);
     if (!bbox_of(v, p.bbox)) { err = "invalid rectangle"; return false; }
     p.v = std::move(v);
-    p.area = signed_area(p.v);     // calculate_poi's helper polygon recomputes the area
+    p.area = area;
+    out = std::move(p);
+    return true;
+}
+
+bool polygon_new(const float* xy, uint32_t n, Polygon& out, std::string& err) {   // simple_polygon.rs:41-71
+    Polygon p;
+    if (!polygon_prepare(xy, n, p, err)) return false;
     float4 poi;
     if (!compute_pole(p, {}, poi)) { err = "no pole found"; return false; }
     p.poi = poi;
-    p.area = area;
     out = std::move(p);
     return true;
 }
@@ -306,13 +278,19 @@ static bool surrogate_piers(const Polygon& shape, uint32_t n, const std::vector<
 
 bool polygon_generate_surrogate(Polygon& p, const jg_surrogate_config& cfg, std::string& err) {  // sp_surrogate.rs:33-53
     if (p.sur.present && std::memcmp(&p.sur.cfg, &cfg, sizeof cfg) == 0) return true;
+    std::vector<float4> poles;
+    if (!surrogate_poles(p, cfg, poles)) { err = "pole generation failed"; return false; }
+    return surrogate_finish(p, cfg, std::move(poles), err);
+}
+
+bool surrogate_finish(Polygon& p, const jg_surrogate_config& cfg, std::vector<float4> poles, std::string& err) {
     Surrogate s;
     std::vector<float2> h = hull_points(p.v);
     for (const float2& q : h)
         for (uint32_t i = 0; i < p.n(); i++)
             if (p.v[i].x == q.x && p.v[i].y == q.y) { s.hull_idx.push_back(i); break; }
     s.hull_area = signed_area(h);
-    if (!surrogate_poles(p, cfg, s.poles)) { err = "pole generation failed"; return false; }
+    s.poles = std::move(poles);
     const size_t n_ff = std::min<size_t>(cfg.n_ff_poles, s.poles.size());
     std::vector<float4> rel(s.poles.begin(), s.poles.begin() + (long)n_ff);
     if (!surrogate_piers(p, cfg.n_ff_piers, rel, s.piers)) { err = "pier generation failed"; return false; }

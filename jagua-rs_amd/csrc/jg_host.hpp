@@ -37,6 +37,11 @@ struct Polygon {
 // polygon construction (simple_polygon.rs, fail_fast/*.rs, convex_hull.rs)
 bool polygon_new(const float* xy, uint32_t n, Polygon& out, std::string& err);
 bool polygon_generate_surrogate(Polygon& p, const jg_surrogate_config& cfg, std::string& err);
+// the two halves around the pole searches, for callers that search poles elsewhere:
+// SPolygon::new without the pole of inaccessibility (poi left unset), and the surrogate from
+// its poles (hull, piers)
+bool polygon_prepare(const float* xy, uint32_t n, Polygon& out, std::string& err);
+bool surrogate_finish(Polygon& p, const jg_surrogate_config& cfg, std::vector<float4> poles, std::string& err);
 Polygon polygon_transform(const Polygon& p, const Mat& m);
 bool polygon_has_point(const Polygon& p, float x, float y);
 Mat mat_from_dt(float rotation, float tx, float ty);

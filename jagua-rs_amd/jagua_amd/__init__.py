@@ -104,6 +104,21 @@ class SPolygon:
         check(N.lib().jg_spolygon_new(_ptr(pts), pts.shape[0], C.byref(h)))
         return cls(h.value)
 
+    @classmethod
+    def new_batch(cls, polygons, surrogate_config=None, with_surrogate=True):
+        """SPolygon::new (+ generate_surrogate) for many polygons, pole searches on the GPU
+        (jg_spolygon_new_batch).  Returns (list of SPolygon, number searched on the host)."""
+        arrs = [_f32(p).reshape(-1, 2) for p in polygons]
+        off = np.zeros(len(arrs) + 1, dtype=np.uint32)
+        off[1:] = np.cumsum([len(a) for a in arrs])
+        xy = np.ascontiguousarray(np.concatenate(arrs) if arrs else np.zeros((0, 2), np.float32))
+        hs = (C.c_void_p * max(1, len(arrs)))()
+        n_host = C.c_uint32()
+        cfg = (surrogate_config or SPSurrogateConfig())._c() if with_surrogate else None
+        check(N.lib().jg_spolygon_new_batch(_ptr(xy), _ptr(off, C.c_uint32), len(arrs),
+                                            C.byref(cfg) if cfg is not None else None, hs, C.byref(n_host)))
+        return [cls(hs[i]) for i in range(len(arrs))], n_host.value
+
     def __del__(self, _N=N):
         if getattr(self, "_h", None) and self._h.value and _N._lib is not None:
             _N._lib.jg_spolygon_free(self._h)

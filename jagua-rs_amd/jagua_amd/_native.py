@@ -65,6 +65,8 @@ _SIGS = {
     "jg_spolygon_new": (C.c_int32, [c_f32p, C.c_uint32, C.POINTER(c_vp)]),
     "jg_spolygon_generate_surrogate": (C.c_int32, [c_vp, C.POINTER(SurrogateConfig)]),
     "jg_spolygon_transform_clone": (C.c_int32, [c_vp, c_f32p, C.POINTER(c_vp)]),
+    "jg_spolygon_new_batch": (C.c_int32, [c_f32p, C.POINTER(C.c_uint32), C.c_uint32, C.POINTER(SurrogateConfig),
+                                          C.POINTER(c_vp), C.POINTER(C.c_uint32)]),
     "jg_spolygon_free": (None, [c_vp]),
     "jg_spolygon_get_info": (C.c_int32, [c_vp, C.POINTER(SPolygonInfoC)]),
     "jg_spolygon_get_vertices": (C.c_int32, [c_vp, c_f32p]),

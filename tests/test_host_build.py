@@ -142,6 +142,11 @@ def test_queries_need_the_gpu():
     item = J.SPolygon.new([[0, 0], [1, 0], [1, 1], [0, 1]]).generate_surrogate()
     with pytest.raises(J.JaguaError, match="no HIP device"):
         cde.detect_batch(item, np.zeros((4, 6), np.float32))
+    # the batched construction validates polygons on the host, then needs the device
+    with pytest.raises(J.JaguaError, match="polygon 1: simple polygon has no area"):
+        J.SPolygon.new_batch([[[0, 0], [1, 0], [0, 1]], [[0, 0], [1, 0], [2, 0]]])
+    with pytest.raises(J.JaguaError, match="no HIP device"):
+        J.SPolygon.new_batch([[[0, 0], [1, 0], [0, 1]]])
 
 
 def test_flattened_tree_records():
