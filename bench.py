@@ -352,7 +352,8 @@ def run_c5(args):
     poles = [p.poles for p in lay]
     off = np.zeros(len(poles) + 1, dtype=np.uint32)
     off[1:] = np.cumsum([len(p) for p in poles])
-    pose = np.stack([dt for _, dt in scene.items]).astype(np.float32)
+    # overlap pose (x, y, theta) from each placed item's DTransformation (rotation, tx, ty)
+    pose = np.stack([(dt[1], dt[2], dt[0]) for _, dt in scene.items]).astype(np.float32)
     rad = np.array([np.max(np.hypot(p[:, 0], p[:, 1]) + p[:, 2]) for p in poles], dtype=np.float32)
     pairs = J.candidate_pairs(pose, rad, [0.0, 0.0, 6000.0, 2000.0], 10)
     cedges = np.array([[0, 0, 6000, 0], [6000, 0, 6000, 2000], [6000, 2000, 0, 2000], [0, 2000, 0, 0]], np.float32)
