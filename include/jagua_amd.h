@@ -231,7 +231,7 @@ typedef struct {
 
 typedef struct {
     uint32_t chunk;                       /* transforms per work unit (0 = autotuned) */
-    uint32_t workgroups;                  /* persistent workgroups (0 = autotuned) */
+    uint32_t workgroups;                  /* persistent one-wavefront workgroups (0 = 16 per compute unit) */
     uint32_t lanes_per_query;             /* 0 = per item, from its vertex count */
     uint32_t flags;                       /* JG_QUERY_* */
 } jg_plan_config;
@@ -245,7 +245,7 @@ jg_status jg_plan_detect_graph(jg_plan* plan, const float* d_mat6, uint8_t* d_st
                                void* stream);
 /* Schedule autotuner (no reference counterpart: the reference evaluates one query at a
  * time).  Times the plan's launch on `stream` with HIP events for candidate schedules —
- * lanes per query in the polygon stage {per item, 8, 16, 32, 64}, workgroups {1, 2, 4, 8}
+ * lanes per query in the polygon stage {per item, 8, 16, 32, 64}, workgroups {4, 8, 16, 32}
  * per compute unit, queries per work unit {16, 32, 64} — by coordinate descent in that
  * order (median of `reps` timed launches each, after one warm-up), and keeps the fastest
  * in the plan.  The plan's flags are kept.  Every schedule computes the same verdicts;

@@ -54,20 +54,18 @@ struct Group {             // 16 B: surviving queries q0 + i for the set bits i 
 // counters: [0] surrogate units taken, [1] groups appended, [2] groups taken
 constexpr int C_UNITS = 0, C_GROUPS = 1, C_TAKEN = 2;
 
-// per-workgroup LDS of the deferred leaf lists (4 waves)
-constexpr uint32_t DEFER_CAP = 512;
-struct DeferLds {
-    uint32_t items[4][DEFER_CAP];
-    uint32_t count[4];
-    float4 geom[4][64];
-    uint32_t kind[4][64];
-    uint64_t qidx[4][64];
-    uint32_t hit[4][64];
+// LDS of a workgroup (one wavefront): its work list
+constexpr uint32_t WAVE = 64;
+struct WaveLds {
+    uint32_t stack[WL_CAP];
+    float4 geom[WL_ENT];
+    uint32_t hit[64];
+    uint32_t qidx[64];
+    uint8_t ent_slot[WL_ENT];
 };
 
-__device__ __forceinline__ Deferred wave_deferred(DeferLds& L) {
-    const uint32_t w = threadIdx.x >> 6;
-    return Deferred{L.items[w], &L.count[w], DEFER_CAP, L.geom[w], L.kind[w], L.qidx[w], L.hit[w]};
+__device__ __forceinline__ WorkList wave_list(WaveLds& L) {
+    return WorkList{L.stack, L.geom, L.ent_slot, L.hit, L.qidx};
 }
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
@@ -82,13 +80,13 @@ __global__ void k_reset_counters(uint32_t* counters) {
 }
 
 template <bool FILT, int KIND>
-__global__ __launch_bounds__(256) void k_surrogate(const SetDesc* __restrict__ sets, const Unit* __restrict__ units,
+__global__ __launch_bounds__(WAVE) void k_surrogate(const SetDesc* __restrict__ sets, const Unit* __restrict__ units,
                                                    uint32_t n_units, uint32_t* __restrict__ counters,
                                                    Group* __restrict__ groups, const float* __restrict__ m6,
                                                    uint8_t* __restrict__ status) {
-    const uint32_t lane = threadIdx.x & 63u;
-    __shared__ DeferLds lds;
-    const Deferred D = wave_deferred(lds);
+    const uint32_t lane = threadIdx.x;
+    __shared__ WaveLds lds;
+    const WorkList W = wave_list(lds);
     for (;;) {
         uint32_t u = 0;
         if (lane == 0) u = atomicAdd(&counters[C_UNITS], 1u);
@@ -107,47 +105,42 @@ __global__ __launch_bounds__(256) void k_surrogate(const SetDesc* __restrict__ s
         uint64_t hit_mask = 0;
         const uint32_t n_ent = (flags & JG_QUERY_SURROGATE) ? it.n_poles + it.n_piers : 0u;
         if (n_ent > 0) {
-            uint32_t g = 1;
-            while (g < n_ent && g < 64u) g <<= 1;     // lanes per query
-            const uint32_t qpp = 64u / g;             // queries per pass
-            for (uint32_t b = 0; b < cnt; b += qpp) {
-                const uint32_t qi = b + lane / g;
-                bool hit = false;
-                // rounds of one entity per lane (a single round unless n_ent > 64); leaves are
-                // deferred to a wave-wide test
-                for (uint32_t e0 = 0; e0 < n_ent; e0 += g) {
-                    if (lane == 0) *D.count = 0u;
-                    D.hit[lane] = 0u;
-                    wave_sync_lds();
-                    const uint32_t e = e0 + lane % g;
-                    bool h = false;
-                    if (qi < cnt && e < n_ent) {
-                        const uint64_t q = (uint64_t)un.q0 + qi;
-                        const Mat m = load_mat(m6, sd.first + q);
-                        const float4 p = it.ff[e];
-                        const bool circle = e < it.n_poles;
-                        float4 gm;
-                        if (circle) {
-                            const float2 cc = xform(m, p.x, p.y);
-                            gm = make_float4(cc.x, cc.y, p.z, 0.0f);
-                        } else {
-                            const float2 a = xform(m, p.x, p.y), bb = xform(m, p.z, p.w);
-                            gm = make_float4(a.x, a.y, bb.x, bb.y);
+            // every (query, ff pole / pier) of the unit descends from the root
+            // (cd_engine.rs:204-223), WL_ENT entities per traversal
+            W.hit[lane] = 0u;
+            if (FILT) W.qidx[lane] = un.q0 + lane;
+            const uint32_t total = cnt * n_ent;
+            for (uint32_t e0 = 0; e0 < total; e0 += WL_ENT) {
+                wave_sync();
+                uint32_t n = 0;
+                for (uint32_t j = 0; j < WL_ENT; j += WAVE) {
+                    const uint32_t o = j + lane;
+                    const uint32_t e = e0 + o;
+                    bool want = false;
+                    if (e < total) {
+                        const uint32_t qi = e / n_ent, k = e - qi * n_ent;
+                        if (!W.hit[qi]) {
+                            const Mat m = load_mat(m6, base + qi);
+                            const float4 p = it.ff[k];
+                            const bool circle = k < it.n_poles;
+                            float4 gm;
+                            if (circle) {
+                                const float2 cc = xform(m, p.x, p.y);
+                                gm = make_float4(cc.x, cc.y, p.z, 0.0f);
+                            } else {
+                                const float2 a = xform(m, p.x, p.y), bb = xform(m, p.z, p.w);
+                                gm = make_float4(a.x, a.y, bb.x, bb.y);
+                            }
+                            W.geom[o] = gm;
+                            W.ent_slot[o] = (uint8_t)(qi | (circle ? WL_CIRCLE : 0u));
+                            want = true;
                         }
-                        D.geom[lane] = gm;
-                        D.kind[lane] = circle ? 1u : 0u;
-                        D.qidx[lane] = q;
-                        h = descend_deferring<FILT, KIND>(c, 0, circle, gm, f, q, lane, D);
                     }
-                    h = test_deferred<FILT, KIND>(c, f, D, lane) || h;
-                    hit = hit || h;
+                    n = wave_push_roots(W, n, want, o, 0u, lane);
                 }
-                const uint64_t bal = ballot(hit);
-                for (uint32_t j = 0; j < qpp && b + j < cnt; j++) {
-                    const uint64_t grp = g == 64 ? ~0ull : (((1ull << g) - 1ull) << (j * g));
-                    if (bal & grp) hit_mask |= 1ull << (b + j);
-                }
+                wave_traverse<FILT, KIND>(c, f, W, n, lane);
             }
+            hit_mask = __ballot(W.hit[lane] != 0u) & valid;
         }
         const uint64_t todo = (flags & JG_QUERY_POLY) ? (valid & ~hit_mask) : 0ull;
         if (lane < cnt) {
@@ -159,16 +152,17 @@ __global__ __launch_bounds__(256) void k_surrogate(const SetDesc* __restrict__ s
             const uint32_t gi = atomicAdd(&counters[C_GROUPS], 1u);
             groups[gi] = Group{un.set, un.q0, todo};
         }
+        wave_sync();
     }
 }
 
 template <bool FILT>
-__global__ __launch_bounds__(256) void k_poly(const SetDesc* __restrict__ sets, uint32_t* __restrict__ counters,
+__global__ __launch_bounds__(WAVE) void k_poly(const SetDesc* __restrict__ sets, uint32_t* __restrict__ counters,
                                               const Group* __restrict__ groups, const float* __restrict__ m6,
                                               uint8_t* __restrict__ status) {
-    const uint32_t lane = threadIdx.x & 63u;
-    __shared__ DeferLds lds;
-    const Deferred D = wave_deferred(lds);
+    const uint32_t lane = threadIdx.x;
+    __shared__ WaveLds lds;
+    const WorkList W = wave_list(lds);
     const uint32_t n_groups = __atomic_load_n(&counters[C_GROUPS], __ATOMIC_RELAXED);
     for (;;) {
         uint32_t gi = 0;
@@ -185,6 +179,7 @@ __global__ __launch_bounds__(256) void k_poly(const SetDesc* __restrict__ sets, 
         const uint32_t slot = lane / G, sub = lane % G;
         const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
         const float4 root_bb = c.nodes[0].bbox;
+        const uint32_t nv = it.n_v;
         uint64_t todo = gr.mask;
         while (todo) {
             // this lane's query: the slot-th remaining survivor
@@ -199,7 +194,7 @@ __global__ __launch_bounds__(256) void k_poly(const SetDesc* __restrict__ sets, 
             // transformed bbox (simple_polygon.rs:122-133), reduced over the group's lanes
             float4 bb = make_float4(3.40282347e+38f, 3.40282347e+38f, -3.40282347e+38f, -3.40282347e+38f);
             if (active)
-                for (uint32_t i = sub; i < it.n_v; i += G) {
+                for (uint32_t i = sub; i < nv; i += G) {
                     const float2 p = xform(m, it.v[i].x, it.v[i].y);
                     bb.x = mn(bb.x, p.x); bb.y = mn(bb.y, p.y); bb.z = mx(bb.z, p.x); bb.w = mx(bb.w, p.y);
                 }
@@ -216,31 +211,38 @@ __global__ __launch_bounds__(256) void k_poly(const SetDesc* __restrict__ sets, 
                 done = true;
             }
             const int vr = active ? virtual_root(c, bb) : 0;
-            for (uint32_t i0 = 0; i0 < it.n_v; i0 += G) {
-                if (ballot(!done) == 0ull) break;
-                const uint32_t i = i0 + sub;
-                bool hit = false;
-                if (lane == 0) *D.count = 0u;
-                D.hit[lane] = 0u;
-                wave_sync_lds();
-                if (!done && i < it.n_v) {
-                    const uint32_t j = i + 1 == it.n_v ? 0 : i + 1;
-                    const float2 a = xform(m, it.v[i].x, it.v[i].y);
-                    const float2 b2 = xform(m, it.v[j].x, it.v[j].y);
-                    const float4 ed = make_float4(a.x, a.y, b2.x, b2.y);
-                    D.geom[lane] = ed;
-                    D.kind[lane] = 0u;
-                    D.qidx[lane] = q;
-                    hit = descend_deferring<FILT, KIND_EDGE>(c, vr, false, ed, f, q, lane, D);
+            // edges of the pass's unfinished queries, descending from their virtual roots
+            // (cd_engine.rs:170-175); a query slot is marked hit as soon as one edge collides
+            wave_sync();
+            if (sub == 0) {
+                W.hit[slot] = done ? 1u : 0u;
+                if (FILT) W.qidx[slot] = (uint32_t)q;
+            }
+            // round R: lane (slot, sub) sets up edges sub + G (2R + j), j < WL_ENT / 64
+            constexpr uint32_t EPL = WL_ENT / WAVE;           // entities per lane per round
+            for (uint32_t i0 = 0; i0 < nv; i0 += EPL * G) {
+                wave_sync();
+                if (__ballot(active && W.hit[slot] == 0u) == 0ull) break;
+                uint32_t n = 0;
+                for (uint32_t j = 0; j < EPL; j++) {
+                    const uint32_t i = i0 + j * G + sub;
+                    const uint32_t o = j * WAVE + lane;
+                    const bool want = active && i < nv && W.hit[slot] == 0u;
+                    if (want) {
+                        const uint32_t k = i + 1 == nv ? 0 : i + 1;
+                        const float2 a = xform(m, it.v[i].x, it.v[i].y);
+                        const float2 b2 = xform(m, it.v[k].x, it.v[k].y);
+                        W.geom[o] = make_float4(a.x, a.y, b2.x, b2.y);
+                        W.ent_slot[o] = (uint8_t)slot;
+                    }
+                    n = wave_push_roots(W, n, want, o, (uint32_t)vr, lane);
                 }
-                hit = test_deferred<FILT, KIND_EDGE>(c, f, D, lane) || hit;
-                const uint64_t bal = ballot(hit);
-                if (!done && ((bal >> (slot * G)) & gmask)) {
-                    res = JG_Q_COLLISION;
-                    done = true;
-                }
+                wave_traverse<FILT, KIND_EDGE>(c, f, W, n, lane);
             }
             // containment over the virtual root's hazards (cd_engine.rs:176-193)
+            wave_sync();
+            done = W.hit[slot] != 0u || done;
+            if (active && W.hit[slot] != 0u && res == JG_Q_CLEAR) res = JG_Q_COLLISION;
             bool hit = false;
             if (!done) {
                 const DevNode vn = c.nodes[vr];
@@ -402,9 +404,10 @@ jg_status plan_build_table(jg_plan* p) {
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     p->n_cus = (uint32_t)cus;
-    const uint32_t want = p->cfg.workgroups ? p->cfg.workgroups : (uint32_t)cus * 4u;
-    p->grid_s = std::max<uint32_t>(1u, std::min<uint32_t>(want, (p->n_units + 3u) / 4u));
-    p->grid_p = std::max<uint32_t>(1u, std::min<uint32_t>(want, (p->n_units + 3u) / 4u));
+    // one wavefront per workgroup, each taking one work unit (or survivor group) at a time
+    const uint32_t want = p->cfg.workgroups ? p->cfg.workgroups : (uint32_t)cus * 16u;
+    p->grid_s = std::max<uint32_t>(1u, std::min<uint32_t>(want, p->n_units));
+    p->grid_p = std::max<uint32_t>(1u, std::min<uint32_t>(want, p->n_units));
     return JG_OK;
 }
 
@@ -415,7 +418,7 @@ jg_status plan_launch(jg_plan* p, const float* d_m6, uint8_t* d_status, hipStrea
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream, counters);
     const SetDesc* sets = (const SetDesc*)base;
     const Unit* units = (const Unit*)(base + p->off_units);
-    const dim3 gs(p->grid_s), gp(p->grid_p), blk(256);
+    const dim3 gs(p->grid_s), gp(p->grid_p), blk(WAVE);
 #define JG_SURR(F, K) hipLaunchKernelGGL((k_surrogate<F, K>), gs, blk, 0, stream, sets, units, p->n_units, counters, \
                                          groups, d_m6, d_status)
     if (p->filtered) {
@@ -572,7 +575,7 @@ extern "C" jg_status jg_plan_autotune(jg_plan* p, const float* d_m6, uint8_t* d_
     if (st == JG_OK) st = plan_time(p, d_m6, d_status, reps, s, a, b, cur_ms);
     const uint32_t cus = p->n_cus ? p->n_cus : 256u;
     const std::vector<uint32_t> lanes = {0u, 8u, 16u, 32u, 64u};
-    const std::vector<uint32_t> wgs = {cus, 2u * cus, 4u * cus, 8u * cus};
+    const std::vector<uint32_t> wgs = {4u * cus, 8u * cus, 16u * cus, 32u * cus};
     const std::vector<uint32_t> chunks = {16u, 32u, 64u};
     for (int dim = 0; dim < 3 && st == JG_OK; dim++) {
         const std::vector<uint32_t>& vals = dim == 0 ? lanes : dim == 1 ? wgs : chunks;

@@ -185,26 +185,34 @@ __device__ __forceinline__ bool collides_edge(const CdeView& c, int start, float
 }
 
 // ---------------------------------------------------------------------------------------
-// Deferred leaf tests.  A wave's lanes descend independently; a lane that reaches a leaf
-// whose strongest relevant hazard is Partial records (lane, leaf) in a per-wave LDS list
-// instead of testing it inline, and the wave then tests all recorded leaves with all of its
-// lanes.  The verdict of a lane is the OR of its Entire hits and its leaf hits, exactly the
-// result of QTNode::collides (qt_node.rs:99-147), whose recursion only decides which hazard
-// is reported first.
+// Wave-cooperative traversal.  QTNode::collides (qt_node.rs:99-147) answers whether any
+// relevant hazard of the subtree collides with the entity; its recursion order only decides
+// which hazard is found first.  A wave therefore turns all descents of its entities (up to
+// WL_ENT circles or edges, each belonging to one query slot) into one LDS stack of
+// (entity, node) visits: each step the 64 lanes pop the top 64 visits, test them, and push
+// the children each visit selects.  A query's verdict is the OR of its entities' Entire hits
+// and leaf hits, exactly the reference's verdict, while every lane stays busy.
+//
+// Kernels using it run one wavefront per workgroup: lanes exchange stack entries and hit flags
+// through LDS only across wave_sync(), a workgroup barrier, which the compiler must reach with
+// all lanes together; the stack depth is a wave-uniform value every lane derives from ballots.
 
-struct Deferred {                     // per-wave LDS
-    uint32_t* items;                  // lane << 26 | leaf node
-    uint32_t* count;
-    uint32_t cap;
-    float4* geom;                     // per lane: circle (cx, cy, r, 1) or edge (x1 y1 x2 y2)
-    uint32_t* kind;                   // per lane: 1 circle, 0 edge
-    uint64_t* qidx;                   // per lane: query index (hashed filters)
-    uint32_t* hit;                    // per lane: leaf hit flag
-};
-
-// Entity kinds of a descent: KIND_EDGE and KIND_CIRCLE are fixed at compile time;
-// KIND_MIXED reads the per-lane kind (surrogates with both poles and piers).
+// Entity kinds of a traversal: KIND_EDGE and KIND_CIRCLE are fixed at compile time;
+// KIND_MIXED reads the per-entity kind (surrogates with both poles and piers).
 enum : int { KIND_EDGE = 0, KIND_CIRCLE = 1, KIND_MIXED = 2 };
+
+constexpr uint32_t WL_ENT = 128;          // entities per traversal
+constexpr uint32_t WL_CAP = 1024;         // stack entries per wave
+constexpr uint32_t WL_NODE_MASK = 0xffffffu;
+constexpr uint32_t WL_CIRCLE = 0x80u;     // WorkList::ent_slot flag: the entity is a circle
+
+struct WorkList {                         // per-wave LDS
+    uint32_t* stack;                      // entity << 24 | node
+    float4* geom;                         // [WL_ENT] circle (cx, cy, r, -) or edge (x1 y1 x2 y2)
+    uint8_t* ent_slot;                    // [WL_ENT] query slot (0..63) | WL_CIRCLE
+    uint32_t* hit;                        // [64] per query slot
+    uint32_t* qidx;                       // [64] per query slot: query index (hashed filters)
+};
 
 template <bool FILT>
 __device__ __forceinline__ bool irrel(const FilterView& f, uint64_t q, uint32_t slot) {
@@ -223,10 +231,9 @@ __device__ __forceinline__ int strongest_t(const CdeView& c, const DevNode& nd, 
 
 // QTHazPartial::collides_with over the relevant Partial entries of a leaf, from entry s on
 // (qt_node.rs:126-141, qt_partial_hazard.rs:67-72).
-template <bool FILT, int KIND>
-__device__ __forceinline__ bool leaf_hits(const CdeView& c, const DevNode& nd, int s, bool circle_rt, float4 g,
+template <bool FILT>
+__device__ __forceinline__ bool leaf_hits(const CdeView& c, const DevNode& nd, int s, bool circle, float4 g,
                                           const FilterView& f, uint64_t q) {
-    const bool circle = KIND == KIND_CIRCLE || (KIND == KIND_MIXED && circle_rt);
     const int he = (int)(nd.haz_begin + (nd.haz_info & COUNT_MASK));
     for (int i = s; i < he; i++) {
         const DevEntry e = c.ent[i];
@@ -244,62 +251,106 @@ __device__ __forceinline__ bool leaf_hits(const CdeView& c, const DevNode& nd, i
     return false;
 }
 
-// Descent of one entity (circle: g = (cx, cy, r, -); edge: g = segment) from `start`,
-// returning true on an Entire hit; leaves are deferred (tested inline when the list is full).
-template <bool FILT, int KIND>
-__device__ __forceinline__ bool descend_deferring(const CdeView& c, int start, bool circle_rt, float4 g,
-                                                  const FilterView& f, uint64_t q, uint32_t lane, const Deferred& D) {
-    const bool circle = KIND == KIND_CIRCLE || (KIND == KIND_MIXED && circle_rt);
-    return descend(c, start, [&](int n, int& fc) -> uint32_t {
-        const DevNode nd = c.nodes[n];
-        bool entire = false;
-        const int s = strongest_t<FILT>(c, nd, f, q, entire);
-        if (s < 0) return 0u;
-        if (entire) return HIT;
-        if (nd.first_child >= 0) {
-            fc = nd.first_child;
-            if (circle) {
-                uint32_t m = 0;
-                for (int k = 0; k < 4; k++) m |= circle_box(g.x, g.y, g.z, box_quadrant(nd.bbox, k)) ? (1u << k) : 0u;
-                return m;
-            }
-            float4 qs[4];
-            for (int k = 0; k < 4; k++) qs[k] = box_quadrant(nd.bbox, k);
-            return edge_quadrants(g, nd.bbox, qs);
+// One visit of QTNode::collides: HIT, or the mask of children to visit (first child in fc).
+template <bool FILT>
+__device__ __forceinline__ uint32_t visit_node(const CdeView& c, int n, bool circle, float4 g, const FilterView& f,
+                                               uint64_t q, int& fc) {
+    const DevNode nd = c.nodes[n];
+    bool entire = false;
+    const int s = strongest_t<FILT>(c, nd, f, q, entire);
+    if (s < 0) return 0u;
+    if (entire) return HIT;
+    if (nd.first_child >= 0) {
+        fc = nd.first_child;
+        if (circle) {
+            uint32_t m = 0;
+            for (int k = 0; k < 4; k++) m |= circle_box(g.x, g.y, g.z, box_quadrant(nd.bbox, k)) ? (1u << k) : 0u;
+            return m;
         }
-        const uint32_t at = atomicAdd(D.count, 1u);
-        if (at < D.cap) {
-            D.items[at] = (lane << 26) | (uint32_t)n;
-            return 0u;
-        }
-        return leaf_hits<FILT, KIND>(c, nd, s, circle_rt, g, f, q) ? HIT : 0u;
-    });
-}
-
-__device__ __forceinline__ void wave_sync_lds() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Tests every deferred leaf with the wave's lanes (round-robin) and returns this lane's flag.
-template <bool FILT, int KIND>
-__device__ __forceinline__ bool test_deferred(const CdeView& c, const FilterView& f, const Deferred& D, uint32_t lane) {
-    wave_sync_lds();
-    const uint32_t n = min(*D.count, D.cap);
-    for (uint32_t i = lane; i < n; i += 64) {
-        const uint32_t it = D.items[i];
-        const uint32_t owner = it >> 26;
-        if (D.hit[owner]) continue;
-        const DevNode nd = c.nodes[it & 0x3ffffffu];
-        const uint64_t q = FILT ? D.qidx[owner] : 0ull;
-        bool entire = false;
-        const int s = strongest_t<FILT>(c, nd, f, q, entire);
-        if (s >= 0 && leaf_hits<FILT, KIND>(c, nd, s, KIND == KIND_MIXED && D.kind[owner] != 0u, D.geom[owner], f, q))
-            D.hit[owner] = 1u;
+        float4 qs[4];
+        for (int k = 0; k < 4; k++) qs[k] = box_quadrant(nd.bbox, k);
+        return edge_quadrants(g, nd.bbox, qs);
     }
-    wave_sync_lds();
-    return D.hit[lane] != 0u;
+    return leaf_hits<FILT>(c, nd, s, circle, g, f, q) ? HIT : 0u;
+}
+
+// The workgroup is a single wavefront: a convergent barrier ordering its lanes' LDS accesses.
+__device__ __forceinline__ void wave_sync() { __syncthreads(); }
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m, uint32_t lane) {
+    return (uint32_t)__builtin_popcountll(m & ((1ull << lane) - 1ull));
+}
+
+// Stack positions for per-lane push counts k in 0..7: exclusive prefix and total over the wave.
+__device__ __forceinline__ uint32_t wave_push_offsets(uint32_t k, uint32_t lane, uint32_t& total) {
+    const uint64_t b0 = __ballot((k & 1u) != 0u), b1 = __ballot((k & 2u) != 0u), b2 = __ballot((k & 4u) != 0u);
+    total = (uint32_t)__builtin_popcountll(b0) + 2u * (uint32_t)__builtin_popcountll(b1) +
+            4u * (uint32_t)__builtin_popcountll(b2);
+    return lanes_below(b0, lane) + 2u * lanes_below(b1, lane) + 4u * lanes_below(b2, lane);
+}
+
+// Pushes one (entity, node) visit per lane with want set onto a stack of depth n (callers set
+// geom/ent_slot first); returns the new depth, the same in every lane.
+__device__ __forceinline__ uint32_t wave_push_roots(const WorkList& W, uint32_t n, bool want, uint32_t ent,
+                                                    uint32_t node, uint32_t lane) {
+    uint32_t total;
+    const uint32_t off = wave_push_offsets(want ? 1u : 0u, lane, total);
+    if (want) W.stack[n + off] = (ent << 24) | node;
+    return n + total;
+}
+
+// Runs the wave's n pending visits to completion; W.hit[slot] is set for every query slot one
+// of whose entities collides.  Callers push at most WL_ENT roots (WL_ENT <= WL_CAP).
+template <bool FILT, int KIND>
+__device__ __forceinline__ void wave_traverse(const CdeView& c, const FilterView& f, const WorkList& W, uint32_t n,
+                                              uint32_t lane) {
+    while (n != 0u) {
+        wave_sync();                              // pushes and hit flags of the last step are visible
+        const uint32_t take = n < 64u ? n : 64u;
+        const uint32_t base = n - take;
+        uint32_t item = 0;
+        const bool have = lane < take;
+        if (have) item = W.stack[n - 1u - lane];
+        uint32_t m = 0;
+        int fc = 0;
+        uint32_t slot = 0;
+        bool circle = KIND == KIND_CIRCLE;
+        float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        uint64_t q = 0;
+        if (have) {
+            const uint32_t es = W.ent_slot[item >> 24];
+            slot = es & 63u;
+            if (!W.hit[slot]) {
+                g = W.geom[item >> 24];
+                if (KIND == KIND_MIXED) circle = (es & WL_CIRCLE) != 0u;
+                if (FILT) q = W.qidx[slot];
+                m = visit_node<FILT>(c, (int)(item & WL_NODE_MASK), circle, g, f, q, fc);
+                if (m == HIT) {
+                    W.hit[slot] = 1u;
+                    m = 0u;
+                }
+            }
+        }
+        uint32_t total;
+        const uint32_t off = wave_push_offsets((uint32_t)__builtin_popcount(m), lane, total);
+        wave_sync();                              // every lane has read its item before pushes overwrite
+        if (base + total <= WL_CAP) {
+            uint32_t at = base + off;
+            for (uint32_t r = m; r; r &= r - 1u)
+                W.stack[at++] = (item & ~WL_NODE_MASK) | (uint32_t)(fc + __builtin_ctz(r));
+            n = base + total;
+        } else {
+            // stack full: these lanes finish their children's subtrees on their own
+            bool h = false;
+            for (uint32_t r = m; r && !h; r &= r - 1u)
+                h = descend(c, fc + __builtin_ctz(r), [&](int nn, int& ffc) -> uint32_t {
+                    return visit_node<FILT>(c, nn, circle, g, f, q, ffc);
+                });
+            if (h) W.hit[slot] = 1u;
+            n = base;
+        }
+    }
+    wave_sync();
 }
 
 // Hazard-polygon point-in-polygon (simple_polygon.rs:255-292) over the hazard's full shape.

@@ -23,7 +23,7 @@ using namespace jg;
 
 namespace {
 
-constexpr int OV_WAVES = 4;   // waves per workgroup
+constexpr int OV_WAVES = 1;   // waves per workgroup: the LDS tile exchange uses the workgroup barrier
 
 __device__ __forceinline__ float wsum(float v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -65,12 +65,12 @@ struct OvView {
 }  // namespace
 
 // ---- forward -------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_ov_fwd_pairs(OvView v, const float* __restrict__ pose3,
+__global__ __launch_bounds__(64 * OV_WAVES) void k_ov_fwd_pairs(OvView v, const float* __restrict__ pose3,
                                                       float* __restrict__ partial) {
     __shared__ float4 lds[OV_WAVES][64];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint64_t pid = (uint64_t)blockIdx.x * OV_WAVES + wave;
-    if (pid >= v.n_pairs) return;   // whole waves exit together; no block barrier below
+    if (pid >= v.n_pairs) return;   // the workgroup is this one wave: it exits as a whole
     const uint2 pr = v.pairs[pid];
     const float3 pa = make_float3(pose3[3 * pr.x], pose3[3 * pr.x + 1], pose3[3 * pr.x + 2]);
     const float3 pb = make_float3(pose3[3 * pr.y], pose3[3 * pr.y + 1], pose3[3 * pr.y + 2]);
@@ -88,8 +88,7 @@ __global__ __launch_bounds__(256) void k_ov_fwd_pairs(OvView v, const float* __r
             const float2 rp = rot(sb, cb, p);
             lds[wave][lane] = make_float4(rp.x, rp.y, p.z, 0.0f);
         }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __syncthreads();
         for (uint32_t i = lane; i < na; i += 64) {
             const float4 p = v.poles[a0 + i];
             const float2 rpi = rot(sa, ca, p);
@@ -104,14 +103,13 @@ __global__ __launch_bounds__(256) void k_ov_fwd_pairs(OvView v, const float* __r
                 }
             }
         }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __syncthreads();
     }
     acc = wsum(acc);
     if (lane == 0) partial[pid] = acc;
 }
 
-__global__ __launch_bounds__(256) void k_ov_fwd_container(OvView v, const float* __restrict__ pose3,
+__global__ __launch_bounds__(64 * OV_WAVES) void k_ov_fwd_container(OvView v, const float* __restrict__ pose3,
                                                           float* __restrict__ partial) {
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t item = blockIdx.x * OV_WAVES + wave;
@@ -172,7 +170,7 @@ __global__ __launch_bounds__(256) void k_ov_reduce2(const double* __restrict__ x
 
 // ---- backward ------------------------------------------------------------------------
 // out[pid] = (dQ/dx_a, dQ/dy_a, dQ/dtheta_a, dQ/dx_b, dQ/dy_b, dQ/dtheta_b)
-__global__ __launch_bounds__(256) void k_ov_bwd_pairs(OvView v, const float* __restrict__ pose3,
+__global__ __launch_bounds__(64 * OV_WAVES) void k_ov_bwd_pairs(OvView v, const float* __restrict__ pose3,
                                                       float* __restrict__ out6) {
     __shared__ float4 lds[OV_WAVES][64];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
@@ -195,8 +193,7 @@ __global__ __launch_bounds__(256) void k_ov_bwd_pairs(OvView v, const float* __r
             const float2 rp = rot(sb, cb, p);
             lds[wave][lane] = make_float4(rp.x, rp.y, p.z, 0.0f);
         }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __syncthreads();
         for (uint32_t i = lane; i < na; i += 64) {
             const float4 p = v.poles[a0 + i];
             const float2 rpi = rot(sa, ca, p);
@@ -221,8 +218,7 @@ __global__ __launch_bounds__(256) void k_ov_bwd_pairs(OvView v, const float* __r
             gya += gcy;
             gta += -gcx * rpi.y + gcy * rpi.x;
         }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __syncthreads();
     }
     gxa = wsum(gxa); gya = wsum(gya); gta = wsum(gta);
     gxb = wsum(gxb); gyb = wsum(gyb); gtb = wsum(gtb);
@@ -232,7 +228,7 @@ __global__ __launch_bounds__(256) void k_ov_bwd_pairs(OvView v, const float* __r
     }
 }
 
-__global__ __launch_bounds__(256) void k_ov_bwd_container(OvView v, const float* __restrict__ pose3,
+__global__ __launch_bounds__(64 * OV_WAVES) void k_ov_bwd_container(OvView v, const float* __restrict__ pose3,
                                                           float* __restrict__ out3) {
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t item = blockIdx.x * OV_WAVES + wave;
