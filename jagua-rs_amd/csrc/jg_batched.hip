@@ -57,15 +57,17 @@ constexpr int C_UNITS = 0, C_GROUPS = 1, C_TAKEN = 2;
 // LDS of a workgroup (one wavefront): its work list
 constexpr uint32_t WAVE = 64;
 struct WaveLds {
-    uint32_t stack[WL_CAP];
+    uint32_t nodes[WL_NCAP];
+    uint32_t edges[WL_ECAP];
     float4 geom[WL_ENT];
     uint32_t hit[64];
     uint32_t qidx[64];
+    uint8_t edge_ent[WL_ECAP];
     uint8_t ent_slot[WL_ENT];
 };
 
 __device__ __forceinline__ WorkList wave_list(WaveLds& L) {
-    return WorkList{L.stack, L.geom, L.ent_slot, L.hit, L.qidx};
+    return WorkList{L.nodes, L.edges, L.edge_ent, L.geom, L.ent_slot, L.hit, L.qidx};
 }
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
@@ -303,6 +305,9 @@ jg_status plan_build_table(jg_plan* p) {
         st = item_upload(const_cast<jg_spolygon*>(s.item));
         if (st != JG_OK) return st;
     }
+    for (auto& s : p->sets)
+        if (s.cde->view.n_nodes >= WL_MAX_NODES)
+            return fail(JG_ERR_CAPACITY, "quadtree of more than 2^24 nodes in a batched plan");
     std::vector<uint64_t> gens;
     for (auto& s : p->sets) gens.push_back(s.cde->generation);
     if (gens == p->generations && p->table.ptr) return JG_OK;

@@ -188,26 +188,36 @@ __device__ __forceinline__ bool collides_edge(const CdeView& c, int start, float
 // Wave-cooperative traversal.  QTNode::collides (qt_node.rs:99-147) answers whether any
 // relevant hazard of the subtree collides with the entity; its recursion order only decides
 // which hazard is found first.  A wave therefore turns all descents of its entities (up to
-// WL_ENT circles or edges, each belonging to one query slot) into one LDS stack of
-// (entity, node) visits: each step the 64 lanes pop the top 64 visits, test them, and push
-// the children each visit selects.  A query's verdict is the OR of its entities' Entire hits
-// and leaf hits, exactly the reference's verdict, while every lane stays busy.
+// WL_ENT circles or edges, each belonging to one query slot) into two LDS queues: node visits
+// (entity, node) and edge tests (entity, hazard edge).  Each step takes 64 items of one kind:
+//   node step: the visit's strongest relevant hazard (qt_hazard_vec.rs:60-63) is Entire -> hit;
+//              an internal node pushes the colliding quadrants (qt_traits.rs); a leaf pushes one
+//              edge test per edge of every relevant Partial hazard whose ff_bbox the entity
+//              collides with (qt_partial_hazard.rs:67-72);
+//   edge step: one Circle-Edge or Edge-Edge test per lane (circle.rs:109-113, edge.rs:162-228).
+// Edge steps run whenever 64 tests are pending, so both kinds of step keep every lane busy.
+// A query's verdict is the OR of its entities' hits, exactly the reference's verdict.
 //
-// Kernels using it run one wavefront per workgroup: lanes exchange stack entries and hit flags
+// Kernels using it run one wavefront per workgroup: lanes exchange queue entries and hit flags
 // through LDS only across wave_sync(), a workgroup barrier, which the compiler must reach with
-// all lanes together; the stack depth is a wave-uniform value every lane derives from ballots.
+// all lanes together; queue depths are wave-uniform values read from lane 63 of scans.
 
 // Entity kinds of a traversal: KIND_EDGE and KIND_CIRCLE are fixed at compile time;
 // KIND_MIXED reads the per-entity kind (surrogates with both poles and piers).
 enum : int { KIND_EDGE = 0, KIND_CIRCLE = 1, KIND_MIXED = 2 };
 
 constexpr uint32_t WL_ENT = 128;          // entities per traversal
-constexpr uint32_t WL_CAP = 1024;         // stack entries per wave
+constexpr uint32_t WL_NCAP = 512;         // node-visit queue entries per wave
+constexpr uint32_t WL_ECAP = 1024;        // edge-test queue entries per wave
+constexpr int WL_EDGE_BITS = 10;          // edge tests one leaf visit pushes: fewer than 2^WL_EDGE_BITS
 constexpr uint32_t WL_NODE_MASK = 0xffffffu;
+constexpr uint32_t WL_MAX_NODES = 1u << 24;
 constexpr uint32_t WL_CIRCLE = 0x80u;     // WorkList::ent_slot flag: the entity is a circle
 
 struct WorkList {                         // per-wave LDS
-    uint32_t* stack;                      // entity << 24 | node
+    uint32_t* nodes;                      // [WL_NCAP] entity << 24 | node
+    uint32_t* edges;                      // [WL_ECAP] hazard edge index
+    uint8_t* edge_ent;                    // [WL_ECAP] entity of the edge test
     float4* geom;                         // [WL_ENT] circle (cx, cy, r, -) or edge (x1 y1 x2 y2)
     uint8_t* ent_slot;                    // [WL_ENT] query slot (0..63) | WL_CIRCLE
     uint32_t* hit;                        // [64] per query slot
@@ -229,6 +239,15 @@ __device__ __forceinline__ int strongest_t(const CdeView& c, const DevNode& nd, 
     return strongest(c, nd, f, q, entire);
 }
 
+// QTHazPartial::collides_with's ff_bbox pre-test (qt_partial_hazard.rs:67-72)
+__device__ __forceinline__ bool ff_pass(const DevEntry& e, bool circle, float4 g) {
+    return circle ? circle_box(g.x, g.y, g.z, e.ff) : box_edge(e.ff, g);
+}
+
+__device__ __forceinline__ bool edge_hit(bool circle, float4 g, float4 he) {
+    return circle ? circle_edge(g.x, g.y, g.z, he) : edge_edge(g, he);
+}
+
 // QTHazPartial::collides_with over the relevant Partial entries of a leaf, from entry s on
 // (qt_node.rs:126-141, qt_partial_hazard.rs:67-72).
 template <bool FILT>
@@ -237,16 +256,9 @@ __device__ __forceinline__ bool leaf_hits(const CdeView& c, const DevNode& nd, i
     const int he = (int)(nd.haz_begin + (nd.haz_info & COUNT_MASK));
     for (int i = s; i < he; i++) {
         const DevEntry e = c.ent[i];
-        if (irrel<FILT>(f, q, e.slot_flags & SLOT_MASK)) continue;
-        if (circle) {
-            if (!circle_box(g.x, g.y, g.z, e.ff)) continue;
-            for (uint32_t j = 0; j < e.edge_count; j++)
-                if (circle_edge(g.x, g.y, g.z, c.edges[e.edge_begin + j])) return true;
-        } else {
-            if (!box_edge(e.ff, g)) continue;
-            for (uint32_t j = 0; j < e.edge_count; j++)
-                if (edge_edge(g, c.edges[e.edge_begin + j])) return true;
-        }
+        if (irrel<FILT>(f, q, e.slot_flags & SLOT_MASK) || !ff_pass(e, circle, g)) continue;
+        for (uint32_t j = 0; j < e.edge_count; j++)
+            if (edge_hit(circle, g, c.edges[e.edge_begin + j])) return true;
     }
     return false;
 }
@@ -277,77 +289,156 @@ __device__ __forceinline__ uint32_t visit_node(const CdeView& c, int n, bool cir
 // The workgroup is a single wavefront: a convergent barrier ordering its lanes' LDS accesses.
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }
 
+// Number of set bits of m below this lane.
 __device__ __forceinline__ uint32_t lanes_below(uint64_t m, uint32_t lane) {
     return (uint32_t)__builtin_popcountll(m & ((1ull << lane) - 1ull));
 }
 
-// Stack positions for per-lane push counts k in 0..7: exclusive prefix and total over the wave.
-__device__ __forceinline__ uint32_t wave_push_offsets(uint32_t k, uint32_t lane, uint32_t& total) {
-    const uint64_t b0 = __ballot((k & 1u) != 0u), b1 = __ballot((k & 2u) != 0u), b2 = __ballot((k & 4u) != 0u);
-    total = (uint32_t)__builtin_popcountll(b0) + 2u * (uint32_t)__builtin_popcountll(b1) +
-            4u * (uint32_t)__builtin_popcountll(b2);
-    return lanes_below(b0, lane) + 2u * lanes_below(b1, lane) + 4u * lanes_below(b2, lane);
+// Exclusive prefix sum over the wave of per-lane counts x < 2^BITS, one ballot per bit;
+// total (the sum) is wave-uniform.
+template <int BITS>
+__device__ __forceinline__ uint32_t wave_prefix(uint32_t x, uint32_t lane, uint32_t& total) {
+    uint32_t off = 0, tot = 0;
+    for (int b = 0; b < BITS; b++) {
+        const uint64_t bb = __ballot((x >> b) & 1u);
+        off += lanes_below(bb, lane) << b;
+        tot += (uint32_t)__builtin_popcountll(bb) << b;
+    }
+    total = tot;
+    return off;
 }
 
-// Pushes one (entity, node) visit per lane with want set onto a stack of depth n (callers set
+// Pushes one node visit per lane with want set onto a node queue of depth n (callers set
 // geom/ent_slot first); returns the new depth, the same in every lane.
 __device__ __forceinline__ uint32_t wave_push_roots(const WorkList& W, uint32_t n, bool want, uint32_t ent,
                                                     uint32_t node, uint32_t lane) {
     uint32_t total;
-    const uint32_t off = wave_push_offsets(want ? 1u : 0u, lane, total);
-    if (want) W.stack[n + off] = (ent << 24) | node;
+    const uint32_t off = wave_prefix<1>(want ? 1u : 0u, lane, total);
+    if (want) W.nodes[n + off] = (ent << 24) | node;
     return n + total;
 }
 
-// Runs the wave's n pending visits to completion; W.hit[slot] is set for every query slot one
-// of whose entities collides.  Callers push at most WL_ENT roots (WL_ENT <= WL_CAP).
+// Runs the wave's nn pending node visits (and the edge tests they create) to completion;
+// W.hit[slot] is set for every query slot one of whose entities collides.  Callers push at
+// most WL_ENT roots (WL_ENT <= WL_NCAP).
 template <bool FILT, int KIND>
-__device__ __forceinline__ void wave_traverse(const CdeView& c, const FilterView& f, const WorkList& W, uint32_t n,
+__device__ __forceinline__ void wave_traverse(const CdeView& c, const FilterView& f, const WorkList& W, uint32_t nn,
                                               uint32_t lane) {
-    while (n != 0u) {
+    uint32_t ne = 0;
+    while (nn + ne != 0u) {
         wave_sync();                              // pushes and hit flags of the last step are visible
-        const uint32_t take = n < 64u ? n : 64u;
-        const uint32_t base = n - take;
-        uint32_t item = 0;
+        if (ne >= 64u || nn == 0u) {
+            // edge step
+            const uint32_t take = ne < 64u ? ne : 64u;
+            if (lane < take) {
+                const uint32_t at = ne - 1u - lane;
+                const uint32_t o = W.edge_ent[at];
+                const uint32_t es = W.ent_slot[o];
+                const uint32_t slot = es & 63u;
+                if (!W.hit[slot]) {
+                    const bool circle = KIND == KIND_CIRCLE || (KIND == KIND_MIXED && (es & WL_CIRCLE) != 0u);
+                    if (edge_hit(circle, W.geom[o], c.edges[W.edges[at]])) W.hit[slot] = 1u;
+                }
+            }
+            ne -= take;
+            continue;
+        }
+        // node step
+        const uint32_t take = nn < 64u ? nn : 64u;
+        const uint32_t base = nn - take;
         const bool have = lane < take;
-        if (have) item = W.stack[n - 1u - lane];
-        uint32_t m = 0;
-        int fc = 0;
+        const uint32_t item = have ? W.nodes[nn - 1u - lane] : 0u;
+        const uint32_t o = item >> 24;
+        uint32_t m = 0, n_edges = 0;
+        int fc = 0, s = -1, he = 0;
         uint32_t slot = 0;
         bool circle = KIND == KIND_CIRCLE;
         float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         uint64_t q = 0;
         if (have) {
-            const uint32_t es = W.ent_slot[item >> 24];
+            const uint32_t es = W.ent_slot[o];
             slot = es & 63u;
             if (!W.hit[slot]) {
-                g = W.geom[item >> 24];
+                g = W.geom[o];
                 if (KIND == KIND_MIXED) circle = (es & WL_CIRCLE) != 0u;
                 if (FILT) q = W.qidx[slot];
-                m = visit_node<FILT>(c, (int)(item & WL_NODE_MASK), circle, g, f, q, fc);
-                if (m == HIT) {
+                const DevNode nd = c.nodes[item & WL_NODE_MASK];
+                bool entire = false;
+                s = strongest_t<FILT>(c, nd, f, q, entire);
+                if (s >= 0 && entire) {
                     W.hit[slot] = 1u;
-                    m = 0u;
+                    s = -1;
+                } else if (s >= 0 && nd.first_child >= 0) {
+                    fc = nd.first_child;
+                    if (circle) {
+                        for (int k = 0; k < 4; k++)
+                            m |= circle_box(g.x, g.y, g.z, box_quadrant(nd.bbox, k)) ? (1u << k) : 0u;
+                    } else {
+                        float4 qs[4];
+                        for (int k = 0; k < 4; k++) qs[k] = box_quadrant(nd.bbox, k);
+                        m = edge_quadrants(g, nd.bbox, qs);
+                    }
+                    s = -1;
+                } else if (s >= 0) {
+                    // leaf: count the edge tests of its relevant Partial hazards
+                    he = (int)(nd.haz_begin + (nd.haz_info & COUNT_MASK));
+                    for (int i = s; i < he; i++) {
+                        const DevEntry e = c.ent[i];
+                        if (!irrel<FILT>(f, q, e.slot_flags & SLOT_MASK) && ff_pass(e, circle, g)) n_edges += e.edge_count;
+                    }
                 }
             }
         }
-        uint32_t total;
-        const uint32_t off = wave_push_offsets((uint32_t)__builtin_popcount(m), lane, total);
+        if (n_edges >= (1u << WL_EDGE_BITS)) {
+            // more edge tests than one push carries: this lane tests its leaf's edges itself
+            bool h = false;
+            for (int i = s; i < he && !h; i++) {
+                const DevEntry e = c.ent[i];
+                if (irrel<FILT>(f, q, e.slot_flags & SLOT_MASK) || !ff_pass(e, circle, g)) continue;
+                for (uint32_t j = 0; j < e.edge_count && !h; j++) h = edge_hit(circle, g, c.edges[e.edge_begin + j]);
+            }
+            if (h) W.hit[slot] = 1u;
+            n_edges = 0;
+        }
+        uint32_t n_tot, e_tot;
+        const uint32_t n_off = wave_prefix<3>((uint32_t)__builtin_popcount(m), lane, n_tot);
+        const uint32_t e_off = wave_prefix<WL_EDGE_BITS>(n_edges, lane, e_tot);
         wave_sync();                              // every lane has read its item before pushes overwrite
-        if (base + total <= WL_CAP) {
-            uint32_t at = base + off;
-            for (uint32_t r = m; r; r &= r - 1u)
-                W.stack[at++] = (item & ~WL_NODE_MASK) | (uint32_t)(fc + __builtin_ctz(r));
-            n = base + total;
+        if (base + n_tot <= WL_NCAP) {
+            uint32_t at = base + n_off;
+            for (uint32_t r = m; r; r &= r - 1u) W.nodes[at++] = (item & ~WL_NODE_MASK) | (uint32_t)(fc + __builtin_ctz(r));
+            nn = base + n_tot;
         } else {
-            // stack full: these lanes finish their children's subtrees on their own
+            // node queue full: these lanes finish their children's subtrees on their own
             bool h = false;
             for (uint32_t r = m; r && !h; r &= r - 1u)
-                h = descend(c, fc + __builtin_ctz(r), [&](int nn, int& ffc) -> uint32_t {
-                    return visit_node<FILT>(c, nn, circle, g, f, q, ffc);
+                h = descend(c, fc + __builtin_ctz(r), [&](int nd2, int& ffc) -> uint32_t {
+                    return visit_node<FILT>(c, nd2, circle, g, f, q, ffc);
                 });
             if (h) W.hit[slot] = 1u;
-            n = base;
+            nn = base;
+        }
+        if (ne + e_tot <= WL_ECAP) {
+            uint32_t at = ne + e_off;
+            if (n_edges)
+                for (int i = s; i < he; i++) {
+                    const DevEntry e = c.ent[i];
+                    if (irrel<FILT>(f, q, e.slot_flags & SLOT_MASK) || !ff_pass(e, circle, g)) continue;
+                    for (uint32_t j = 0; j < e.edge_count; j++, at++) {
+                        W.edges[at] = e.edge_begin + j;
+                        W.edge_ent[at] = (uint8_t)o;
+                    }
+                }
+            ne += e_tot;
+        } else if (n_edges) {
+            // edge queue full: this lane tests its leaf's edges itself
+            bool h = false;
+            for (int i = s; i < he && !h; i++) {
+                const DevEntry e = c.ent[i];
+                if (irrel<FILT>(f, q, e.slot_flags & SLOT_MASK) || !ff_pass(e, circle, g)) continue;
+                for (uint32_t j = 0; j < e.edge_count && !h; j++) h = edge_hit(circle, g, c.edges[e.edge_begin + j]);
+            }
+            if (h) W.hit[slot] = 1u;
         }
     }
     wave_sync();

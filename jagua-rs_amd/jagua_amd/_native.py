@@ -116,7 +116,8 @@ _lib = None
 
 
 def lib_path():
-    return os.path.join(LIB_DIR, "libjagua_amd.so")
+    """The in-tree build, or the library named by JAGUA_AMD_LIB (another build of the same ABI)."""
+    return os.environ.get("JAGUA_AMD_LIB") or os.path.join(LIB_DIR, "libjagua_amd.so")
 
 
 def lib():
