@@ -159,6 +159,7 @@ def run_c2(args):
             "plain_matches_batched": plain_ok, "graph_matches_batched": graph_ok}
         log(f"plain: {ms_plain:.3f} ms/step; graph: {ms_graph:.3f} ms/step; same verdicts: {plain_ok and graph_ok}")
         result["overlap"] = overlap_c4(stream)
+        result["search"] = search_c2(cde, items, stream)
 
     roof, cpu = None, None
     if not args.no_cpu:
@@ -191,6 +192,22 @@ def run_c2(args):
     result["roofline"] = roof
     result["cpu_baseline"] = cpu
     return result
+
+
+def search_c2(cde, items, stream):
+    """lbf's placement search for each of the query items on the layout, all at once on the
+    GPU: 5000 samples per item (LBFConfig::default) as 8 uniform and 2 local rounds of 500,
+    continuous rotation, one captured graph per run."""
+    import jagua_amd as J
+    search = J.PlacementSearch([(cde, it, "continuous", None) for it in items], samples_per_round=500,
+                               uniform_rounds=8, local_rounds=2, seed=1)
+    search.run(graph=True)
+    ms = timed(lambda: search.run(graph=True), 10, stream)
+    res = search.results()
+    n_samples = sum(r["samples"] for r in res)
+    log(f"placement search: {len(items)} items x 5000 samples in {ms:.3f} ms, {sum(r['found'] for r in res)} placed")
+    return {"items": len(items), "samples_per_item": 5000, "ms_per_run": ms, "searches_per_s": len(items) / (ms / 1e3),
+            "samples_per_s": n_samples / (ms / 1e3), "placed": int(sum(r["found"] for r in res)), "form": "hipGraph"}
 
 
 def overlap_c4(stream):

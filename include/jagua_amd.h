@@ -256,6 +256,72 @@ jg_status jg_plan_autotune(jg_plan* plan, const float* d_mat6, uint8_t* d_status
  * means per item). */
 jg_status jg_plan_get_config(const jg_plan* plan, jg_plan_config* out);
 
+/* ---------------------------------------------------------------- placement search */
+
+/* lbf's sampled placement search (lbf/src/opt/search.rs:17-106), resident on the GPU for many
+ * independent (engine, item) searches at once.  No reference counterpart as an API: the
+ * reference evaluates one sample at a time and tightens its sampling box after every
+ * improvement.  Here every round draws samples_per_round transforms per search on the device,
+ * evaluates them with the batched collision query (surrogate || polygon, bit-exact), and one
+ * wave per search keeps the clear sample of lowest LBFLoss (x_max * 10 + y_max of the placed
+ * shape's bbox, lbf/src/opt/loss.rs:29-41; ties to the first sample) if it beats the best so
+ * far.  Within a round that is exactly the sequential rule applied to the same samples.
+ *   uniform rounds: rotation uniform over the search's rotation table, translation uniform in
+ *                   the sampling box, initially the CDE bbox; after each round x_max of the box
+ *                   is tightened to best cost / 10 (LBFLoss::tighten_sample_bbox);
+ *   local rounds:   only for searches that found a placement: translation normal around the
+ *                   best with standard deviation decaying from ls_sd_transl[0] to [1] times
+ *                   the larger side of the CDE bbox over the rounds (ls_sampler.rs:8-12,
+ *                   80-95); rotation: discrete tables keep the best's rotation, continuous
+ *                   tables step the best's angle by a normal with deviation ls_sd_rot[0..1].
+ * Rotations come from a host-built table (sin/cos with libm, as DTransformation::compose), so
+ * the device never evaluates trigonometry: each sample's matrix equals the host composition of
+ * its (rotation, tx, ty).  Continuous rotation uses JG_SEARCH_CONT_ROTATIONS angles
+ * 2 pi k / JG_SEARCH_CONT_ROTATIONS.  Random numbers are a counter-based hash of
+ * (seed, search, round, sample): a run is reproducible. */
+#define JG_SEARCH_CONT_ROTATIONS 4096u
+
+typedef struct {
+    uint32_t samples_per_round;           /* per search and round, 1 .. 65536 */
+    uint32_t uniform_rounds;
+    uint32_t local_rounds;
+    uint32_t pad;
+    uint64_t seed;
+    float ls_sd_transl[2];                /* 0.01, 0.0005 in lbf */
+    float ls_sd_rot[2];                   /* 2 and 0.5 degrees in lbf, radians */
+} jg_search_config;
+
+typedef struct {
+    jg_cde* cde;
+    const jg_spolygon* item;              /* with its surrogate */
+    const float* rotations;               /* allowed rotations (radians), n_rotations of them */
+    uint32_t n_rotations;                 /* 0 with continuous = 0: no rotation */
+    uint32_t continuous;                  /* 1: continuous rotation (RotationRange::Continuous) */
+    jg_hazard_filter filter;
+} jg_search_item;
+
+typedef struct {
+    uint32_t found;
+    float rotation, tx, ty;               /* best DTransformation */
+    float cost, x_max, y_max;             /* its LBFLoss */
+    uint32_t samples;                     /* samples drawn for this search */
+} jg_search_result;
+
+typedef struct jg_search jg_search;
+
+jg_status jg_search_create(const jg_search_item* items, uint32_t n_items, const jg_search_config* cfg,
+                           jg_search** out);
+void jg_search_free(jg_search* s);
+/* All rounds of all searches on `stream`; graph = 1 captures the run in a hipGraph once and
+ * replays it on later calls.  Results are valid after the stream synchronises. */
+jg_status jg_search_run(jg_search* s, uint32_t graph, void* stream);
+/* Copies the results of the last run (synchronises). */
+jg_status jg_search_results(jg_search* s, jg_search_result* out);
+/* The last run's samples: dt[(round * n_items + item) * samples_per_round + j][3] (rotation,
+ * tx, ty) and their statuses (JG_Q_*), host arrays of rounds * n_items * samples_per_round
+ * entries (synchronises). */
+jg_status jg_search_samples(jg_search* s, float* dt, uint8_t* status);
+
 /* ---------------------------------------------------------------- overlap gradient */
 
 /* Pole-pair overlap quantity (an extension with no reference counterpart; its

@@ -443,6 +443,15 @@ jg_status plan_launch(jg_plan* p, const float* d_m6, uint8_t* d_status, hipStrea
 
 }  // namespace
 
+namespace jg {
+// for other batched forms built on a plan (the placement search): table upload outside any
+// graph capture, then launches only
+jg_status plan_prepare(jg_plan* p) { return plan_build_table(p); }
+jg_status plan_enqueue(jg_plan* p, const float* d_m6, uint8_t* d_status, hipStream_t stream) {
+    return plan_launch(p, d_m6, d_status, stream);
+}
+}  // namespace jg
+
 extern "C" jg_status jg_plan_create(const jg_query_set* sets, uint32_t n_sets, const jg_plan_config* cfg, jg_plan** out) {
     if (!out || (n_sets && !sets)) return fail(JG_ERR_INVALID_ARG, "null argument");
     auto p = std::make_unique<jg_plan>();

@@ -54,4 +54,6 @@ namespace jg {
 jg_status cde_upload(jg_cde* c);
 jg_status item_upload(jg_spolygon* it);
 jg_status filter_view(jg_cde* c, const jg_hazard_filter* f, FilterView& out);
+jg_status plan_prepare(jg_plan* p);
+jg_status plan_enqueue(jg_plan* p, const float* d_m6, uint8_t* d_status, hipStream_t stream);
 }  // namespace jg

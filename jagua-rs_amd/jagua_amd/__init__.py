@@ -486,6 +486,72 @@ class QueryPlan:
                 "flags": c.flags}
 
 
+class PlacementSearch:
+    """lbf's sampled placement search (lbf/src/opt/search.rs:17-106) for many independent
+    (engine, item) pairs at once, resident on the GPU (jg_search_*, include/jagua_amd.h).
+
+    searches: list of (cde, item, rotations, filter); rotations is None (no rotation),
+    "continuous", or a list of allowed rotations in radians; filter may be None.  Defaults
+    draw lbf's 5000 samples per item (LBFConfig::default, lbf/src/config.rs:33-41) as 8 uniform
+    and 2 local rounds of 500."""
+
+    def __init__(self, searches, samples_per_round=500, uniform_rounds=8, local_rounds=2, seed=0,
+                 ls_sd_transl=(0.01, 0.0005), ls_sd_rot=(2.0 * np.pi / 180.0, 0.5 * np.pi / 180.0)):
+        n = len(searches)
+        arr = (N.SearchItemC * n)()
+        self._keep = []
+        self.n = n
+        for i, (cde, item, rots, filt) in enumerate(searches):
+            arr[i].cde = cde._h.value
+            arr[i].item = item._h.value
+            if isinstance(rots, str):
+                if rots != "continuous":
+                    raise ValueError("rotations: None, 'continuous' or a list of radians")
+                arr[i].continuous = 1
+            elif rots is not None and len(rots):
+                r = _f32(rots).reshape(-1)
+                self._keep.append(r)
+                arr[i].rotations = _ptr(r)
+                arr[i].n_rotations = len(r)
+            f = filt or NoFilter()
+            self._keep.append(f)
+            arr[i].filter = f._filter()
+        cfg = N.SearchConfigC()
+        cfg.samples_per_round, cfg.uniform_rounds, cfg.local_rounds = samples_per_round, uniform_rounds, local_rounds
+        cfg.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        cfg.ls_sd_transl[0], cfg.ls_sd_transl[1] = ls_sd_transl
+        cfg.ls_sd_rot[0], cfg.ls_sd_rot[1] = ls_sd_rot
+        self.samples_per_round = samples_per_round
+        self.rounds = uniform_rounds + local_rounds
+        self.uniform_rounds = uniform_rounds
+        self._refs = [s[0] for s in searches] + [s[1] for s in searches]
+        self._h = C.c_void_p()
+        check(N.lib().jg_search_create(arr, n, C.byref(cfg), C.byref(self._h)))
+
+    def __del__(self, _N=N):
+        if getattr(self, "_h", None) and self._h.value and _N._lib is not None:
+            _N._lib.jg_search_free(self._h)
+            self._h = C.c_void_p()
+
+    def run(self, graph=False, stream=None):
+        check(N.lib().jg_search_run(self._h, 1 if graph else 0, _stream_ptr(stream)))
+        return self
+
+    def results(self):
+        out = (N.SearchResultC * self.n)()
+        check(N.lib().jg_search_results(self._h, out))
+        return [{"found": bool(r.found), "rotation": r.rotation, "tx": r.tx, "ty": r.ty, "cost": r.cost,
+                 "x_max": r.x_max, "y_max": r.y_max, "samples": r.samples} for r in out]
+
+    def samples(self):
+        """(dt [rounds, n, K, 3], status [rounds, n, K]) of the last run."""
+        shape = (self.rounds, self.n, self.samples_per_round)
+        dt = np.empty(shape + (3,), dtype=np.float32)
+        st = np.empty(shape, dtype=np.uint8)
+        check(N.lib().jg_search_samples(self._h, _ptr(dt), _ptr(st, C.c_uint8)))
+        return dt, st
+
+
 class PoleOverlap:
     """The pole-pair overlap quantity Q (include/jagua_amd.h, jg_overlap_*) as a forward /
     backward op pair over per-item poses (x, y, theta), on the GPU.

@@ -60,6 +60,21 @@ class PlanConfigC(C.Structure):
                 ("flags", C.c_uint32)]
 
 
+class SearchConfigC(C.Structure):
+    _fields_ = [("samples_per_round", C.c_uint32), ("uniform_rounds", C.c_uint32), ("local_rounds", C.c_uint32),
+                ("pad", C.c_uint32), ("seed", C.c_uint64), ("ls_sd_transl", C.c_float * 2), ("ls_sd_rot", C.c_float * 2)]
+
+
+class SearchItemC(C.Structure):
+    _fields_ = [("cde", c_vp), ("item", c_vp), ("rotations", c_f32p), ("n_rotations", C.c_uint32),
+                ("continuous", C.c_uint32), ("filter", FilterC)]
+
+
+class SearchResultC(C.Structure):
+    _fields_ = [("found", C.c_uint32), ("rotation", C.c_float), ("tx", C.c_float), ("ty", C.c_float),
+                ("cost", C.c_float), ("x_max", C.c_float), ("y_max", C.c_float), ("samples", C.c_uint32)]
+
+
 # name -> (restype, argtypes)
 _SIGS = {
     "jg_spolygon_new": (C.c_int32, [c_f32p, C.c_uint32, C.POINTER(c_vp)]),
@@ -98,6 +113,11 @@ _SIGS = {
     "jg_plan_autotune": (C.c_int32, [c_vp, c_vp, c_vp, C.c_uint32, c_vp, C.POINTER(PlanConfigC),
                                      C.POINTER(C.c_float)]),
     "jg_plan_get_config": (C.c_int32, [c_vp, C.POINTER(PlanConfigC)]),
+    "jg_search_create": (C.c_int32, [C.POINTER(SearchItemC), C.c_uint32, C.POINTER(SearchConfigC), C.POINTER(c_vp)]),
+    "jg_search_free": (None, [c_vp]),
+    "jg_search_run": (C.c_int32, [c_vp, C.c_uint32, c_vp]),
+    "jg_search_results": (C.c_int32, [c_vp, C.POINTER(SearchResultC)]),
+    "jg_search_samples": (C.c_int32, [c_vp, c_f32p, c_u8p]),
     "jg_overlap_create": (C.c_int32, [c_f32p, c_u32p, C.c_uint32, c_u32p, C.c_uint64, c_f32p, C.c_uint32,
                                       C.POINTER(c_vp)]),
     "jg_overlap_set_pairs": (C.c_int32, [c_vp, c_u32p, C.c_uint64]),
