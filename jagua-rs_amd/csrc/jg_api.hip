@@ -242,14 +242,29 @@ __global__ __launch_bounds__(256) void k_detect_plain(CdeView c, ItemView it, co
 
 namespace {
 
-__device__ __forceinline__ bool set_has(const uint32_t* set, uint32_t slot) { return (set[slot >> 5] >> (slot & 31u)) & 1u; }
-__device__ __forceinline__ void set_put(uint32_t* set, uint32_t slot) { atomicOr(&set[slot >> 5], 1u << (slot & 31u)); }
+// The collector of one query: a bitmask over hazard slots (union semantics) plus the list of
+// slots in insertion order, so that results and the reset cost O(set size), not O(slots).
+constexpr uint32_t SET_LIST_CAP = 256;
+struct SlotSet {
+    uint32_t* bits;
+    uint32_t* list;                 // [SET_LIST_CAP]
+    uint32_t* n;                    // slots inserted (may exceed SET_LIST_CAP: then bits is scanned)
+};
+__device__ __forceinline__ bool set_has(const SlotSet& set, uint32_t slot) {
+    return (set.bits[slot >> 5] >> (slot & 31u)) & 1u;
+}
+__device__ __forceinline__ void set_put(const SlotSet& set, uint32_t slot) {
+    const uint32_t bit = 1u << (slot & 31u);
+    if (atomicOr(&set.bits[slot >> 5], bit) & bit) return;
+    const uint32_t at = atomicAdd(set.n, 1u);
+    if (at < SET_LIST_CAP) set.list[at] = slot;
+}
 
 // Hazard tests of QTNode::collect_collisions at a node where detection is performed now
 // (qt_node.rs:174-189), for a circle or an edge.
 template <bool CIRCLE>
 __device__ __forceinline__ void collect_now(const CdeView& c, const DevNode& nd, float4 ent_geom, float r,
-                                            uint32_t* set) {
+                                            const SlotSet& set) {
     const int he = (int)(nd.haz_begin + (nd.haz_info & COUNT_MASK));
     for (int i = (int)nd.haz_begin; i < he; i++) {
         const DevEntry e = c.ent[i];
@@ -270,7 +285,7 @@ __device__ __forceinline__ void collect_now(const CdeView& c, const DevNode& nd,
 
 // QTNode::collect_collisions (qt_node.rs:151-191)
 template <bool CIRCLE>
-__device__ __forceinline__ void collect_from(const CdeView& c, int start, float4 g, float r, uint32_t* set) {
+__device__ __forceinline__ void collect_from(const CdeView& c, int start, float4 g, float r, const SlotSet& set) {
     descend(c, start, [&](int n, int& fc) -> uint32_t {
         const DevNode nd = c.nodes[n];
         const bool now = nd.n_active_edges <= c.cd_threshold;
@@ -314,24 +329,25 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 
 }  // namespace
 
-// One hazard-collection query per wavefront; the collector is a bitmask over hazard slots in
-// LDS (union semantics of BasicHazardCollector, collector.rs:44-67).  Lanes take the
-// surrogate poles/piers and the query edges; containment checks follow (cd_engine.rs:275-288).
-__global__ __launch_bounds__(256) void k_collect(CdeView c, ItemView it, const float* __restrict__ m6, uint64_t n,
-                                                 uint32_t words, uint32_t cap, uint32_t* __restrict__ count,
-                                                 uint64_t* __restrict__ hash, uint32_t* __restrict__ keys) {
+// Hazard-collection queries, one at a time per wavefront (persistent one-wavefront workgroups,
+// so the workgroup barrier orders the lanes' LDS accesses); the collector is a SlotSet in LDS
+// (union semantics of BasicHazardCollector, collector.rs:44-67).  Lanes take the surrogate
+// poles/piers and the query edges, each collecting its descent (collect_from); containment
+// checks follow (cd_engine.rs:261-310).  A wave traversal with shared queues, as the batched
+// detection uses, measured slower here: collection descents start at deep virtual roots.
+__global__ __launch_bounds__(64) void k_collect(CdeView c, ItemView it, const float* __restrict__ m6, uint64_t n,
+                                                uint32_t words, uint32_t cap, uint32_t* __restrict__ count,
+                                                uint64_t* __restrict__ hash, uint32_t* __restrict__ keys) {
     extern __shared__ uint32_t lds[];
-    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    const uint64_t q = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wave;
-    uint32_t* set = lds + (size_t)wave * words;
-    for (uint32_t w = lane; w < words; w += 64) set[w] = 0u;
+    __shared__ uint32_t list[SET_LIST_CAP];
+    __shared__ uint32_t list_n;
+    const uint32_t lane = threadIdx.x;
+    const SlotSet set{lds, list, &list_n};
+    for (uint32_t w = lane; w < words; w += 64) lds[w] = 0u;
+    if (lane == 0) list_n = 0u;
     __syncthreads();
-    const bool active = q < n;
-    Mat m{};
-    float4 qbb = make_float4(0, 0, 0, 0);
-    int vr = 0;
-    if (active) {
-        m = load_mat(m6, q);
+    for (uint64_t q = blockIdx.x; q < n; q += gridDim.x) {
+        const Mat m = load_mat(m6, q);
         const uint32_t n_sur = it.n_poles + it.n_piers;
         for (uint32_t i = lane; i < n_sur; i += 64) {
             const float4 p = it.ff[i];
@@ -348,59 +364,84 @@ __global__ __launch_bounds__(256) void k_collect(CdeView c, ItemView it, const f
             const float2 p = xform(m, it.v[i].x, it.v[i].y);
             bb.x = mn(bb.x, p.x); bb.y = mn(bb.y, p.y); bb.z = mx(bb.z, p.x); bb.w = mx(bb.w, p.y);
         }
-        qbb = make_float4(wave_min(bb.x), wave_min(bb.y), wave_max(bb.z), wave_max(bb.w));
+        const float4 qbb = make_float4(wave_min(bb.x), wave_min(bb.y), wave_max(bb.z), wave_max(bb.w));
         if (box_relation(c.nodes[0].bbox, qbb) != REL_SURROUNDING && lane == 0) set_put(set, c.exterior_slot);
-        vr = virtual_root(c, qbb);
+        const int vr = virtual_root(c, qbb);
         for (uint32_t i = lane; i < it.n_v; i += 64) {
             const float2 a = xform(m, it.v[i].x, it.v[i].y);
             const uint32_t j = i + 1 == it.n_v ? 0 : i + 1;
             const float2 b = xform(m, it.v[j].x, it.v[j].y);
             collect_from<false>(c, vr, make_float4(a.x, a.y, b.x, b.y), 0.0f, set);
         }
-    }
-    __syncthreads();
-    if (active) {
-        const DevNode vn = c.nodes[vr];
-        const float2 qpoi = xform(m, it.poi.x, it.poi.y);
-        const int he = (int)(vn.haz_begin + (vn.haz_info & COUNT_MASK));
-        for (int i = (int)vn.haz_begin + (int)lane; i < he; i += 64) {
-            const uint32_t sf = c.ent[i].slot_flags;
-            if (sf & ENTIRE_BIT) continue;
-            const uint32_t slot = sf & SLOT_MASK;
-            if (set_has(set, slot)) continue;
-            if (containment(c, slot, it, m, qbb, qpoi)) set_put(set, slot);
+        __syncthreads();
+        {
+            const DevNode vn = c.nodes[vr];
+            const float2 qpoi = xform(m, it.poi.x, it.poi.y);
+            const int he = (int)(vn.haz_begin + (vn.haz_info & COUNT_MASK));
+            for (int i = (int)vn.haz_begin + (int)lane; i < he; i += 64) {
+                const uint32_t sf = c.ent[i].slot_flags;
+                if (sf & ENTIRE_BIT) continue;
+                const uint32_t slot = sf & SLOT_MASK;
+                if (set_has(set, slot)) continue;
+                if (containment(c, slot, it, m, qbb, qpoi)) set_put(set, slot);
+            }
         }
-    }
-    __syncthreads();
-    if (!active) return;
-    uint32_t cnt = 0;
-    uint64_t h = 0;
-    for (uint32_t w = lane; w < words; w += 64) {
-        uint32_t bits = set[w];
-        cnt += (uint32_t)__builtin_popcount(bits);
-        while (bits) {
-            const uint32_t b = (uint32_t)__builtin_ctz(bits);
-            bits &= bits - 1;
-            h += mix64((uint64_t)(w * 32u + b));
-        }
-    }
-    cnt = wave_sum_u32(cnt);
-    h = wave_sum_u64(h);
-    if (lane == 0) {
-        count[q] = cnt;
-        hash[q] = h;
-        if (keys && cap) {
-            uint32_t k = 0;
-            for (uint32_t w = 0; w < words && k < cap; w++) {
-                uint32_t bits = set[w];
-                while (bits && k < cap) {
-                    const uint32_t b = (uint32_t)__builtin_ctz(bits);
-                    bits &= bits - 1;
-                    keys[q * cap + k++] = w * 32u + b;
+        __syncthreads();
+        // results: count, order-independent hash, the first `cap` slots ascending
+        const uint32_t total = list_n;
+        uint32_t cnt = 0;
+        uint64_t h = 0;
+        if (total <= SET_LIST_CAP) {
+            for (uint32_t i = lane; i < total; i += 64) {
+                const uint32_t si = list[i];
+                h += mix64((uint64_t)si);
+                if (keys && cap) {
+                    uint32_t rank = 0;
+                    for (uint32_t j = 0; j < total; j++) rank += list[j] < si ? 1u : 0u;
+                    if (rank < cap) keys[q * cap + rank] = si;
                 }
             }
-            for (; k < cap; k++) keys[q * cap + k] = 0xffffffffu;
+            cnt = total;
+            if (keys)
+                for (uint32_t k = total + lane; k < cap; k += 64) keys[q * cap + k] = 0xffffffffu;
+        } else {
+            for (uint32_t w = lane; w < words; w += 64) {
+                uint32_t bits = lds[w];
+                cnt += (uint32_t)__builtin_popcount(bits);
+                while (bits) {
+                    const uint32_t b = (uint32_t)__builtin_ctz(bits);
+                    bits &= bits - 1;
+                    h += mix64((uint64_t)(w * 32u + b));
+                }
+            }
+            cnt = wave_sum_u32(cnt);
+            if (lane == 0 && keys && cap) {
+                uint32_t k = 0;
+                for (uint32_t w = 0; w < words && k < cap; w++) {
+                    uint32_t bits = lds[w];
+                    while (bits && k < cap) {
+                        const uint32_t b = (uint32_t)__builtin_ctz(bits);
+                        bits &= bits - 1;
+                        keys[q * cap + k++] = w * 32u + b;
+                    }
+                }
+                for (; k < cap; k++) keys[q * cap + k] = 0xffffffffu;
+            }
         }
+        h = wave_sum_u64(h);
+        if (lane == 0) {
+            count[q] = cnt;
+            hash[q] = h;
+        }
+        __syncthreads();
+        // reset the collector for the next query
+        if (total <= SET_LIST_CAP) {
+            for (uint32_t i = lane; i < total; i += 64) lds[list[i] >> 5] = 0u;
+        } else {
+            for (uint32_t w = lane; w < words; w += 64) lds[w] = 0u;
+        }
+        if (lane == 0) list_n = 0u;
+        __syncthreads();
     }
 }
 
@@ -698,12 +739,12 @@ extern "C" jg_status jg_collect_batch(jg_cde* c, const jg_spolygon* item, const 
     s = item_upload(const_cast<jg_spolygon*>(item));
     if (s != JG_OK) return s;
     const uint32_t words = (c->view.n_slots + 31u) / 32u;
-    uint32_t waves = 4;
-    while (waves > 1 && (size_t)waves * words * 4 > 65536) waves >>= 1;
-    if ((size_t)waves * words * 4 > 65536) return fail(JG_ERR_CAPACITY, "too many hazard slots for the LDS collector");
-    const uint64_t grid = (n + waves - 1) / waves;
-    if (grid > 0x7fffffffull) return fail(JG_ERR_INVALID_ARG, "batch too large");
-    hipLaunchKernelGGL(k_collect, dim3((uint32_t)grid), dim3(64 * waves), (size_t)waves * words * 4, (hipStream_t)stream,
+    if ((size_t)words * 4 > 48000) return fail(JG_ERR_CAPACITY, "too many hazard slots for the LDS collector");
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const uint64_t grid = std::min<uint64_t>(n, (uint64_t)cus * 16u);     // persistent one-wave workgroups
+    hipLaunchKernelGGL(k_collect, dim3((uint32_t)grid), dim3(64), (size_t)words * 4, (hipStream_t)stream,
                        c->view, item->view, d_m6, n, words, d_keys ? capacity : 0u, d_count, d_hash, d_keys);
     HIP_TRY(hipGetLastError());
     return JG_OK;

@@ -136,3 +136,15 @@ def test_update_sequence(cuda):
     cde.restore(snap)
     O.lib().or_cde_restore(oc.h, osnap)
     run_both(cde, oc, item, oitem, mats, cuda=cuda)
+
+
+def test_collect_sets_beyond_the_slot_list(cuda):
+    """Hazard sets of more than 256 slots (the collector's slot list in LDS) are counted, hashed
+    and listed exactly through the bitmask scan."""
+    sc = S.config2_scene(seed=14, n_items=1200)
+    cde, keys, oc = engines(sc)
+    poly = S.polygon(21, 64, 64, 600.0, 800.0, 0)
+    (item,), (oitem,) = items([poly])
+    dts = S.transforms(22, 48, (500, 400, 3500, 1100), 0)
+    st, ref = run_both(cde, oc, item, oitem, J.compose(dts), cap=300, cuda=cuda)
+    assert ref["count"].max() > 256 and ref["count"].min() < 256
