@@ -160,6 +160,7 @@ def run_c2(args):
         log(f"plain: {ms_plain:.3f} ms/step; graph: {ms_graph:.3f} ms/step; same verdicts: {plain_ok and graph_ok}")
         result["overlap"] = overlap_c4(stream)
         result["search"] = search_c2(cde, items, stream)
+        result["update"] = update_c2(cde, scene, st)
 
     roof, cpu = None, None
     if not args.no_cpu:
@@ -208,6 +209,49 @@ def search_c2(cde, items, stream):
     log(f"placement search: {len(items)} items x 5000 samples in {ms:.3f} ms, {sum(r['found'] for r in res)} placed")
     return {"items": len(items), "samples_per_item": 5000, "ms_per_run": ms, "searches_per_s": len(items) / (ms / 1e3),
             "samples_per_s": n_samples / (ms / 1e3), "placed": int(sum(r["found"] for r in res)), "form": "hipGraph"}
+
+
+def update_c2(cde, scene, st0, n=100):
+    """The engine updates of lbf placement steps on the 2000-polygon layout: n random items
+    are each removed and placed again at a random pose (deregister + register), with the
+    device image brought up to date (a patch of the touched nodes and slots) after every
+    change; st0 holds the full upload of the same engine for comparison."""
+    import numpy as np
+
+    import jagua_amd as J
+    rng = np.random.default_rng(5)
+    idx = rng.choice(len(scene.items), n, replace=False)
+    (x0, y0), (x1, y1) = scene.container.min(0), scene.container.max(0)
+    poses = np.stack([rng.uniform(0, 2 * np.pi, n), rng.uniform(x0, x1, n), rng.uniform(y0, y1, n)], 1)
+    poses = poses.astype(np.float32)
+    shapes = [J.SPolygon.new(scene.items[i][0]).transform_clone(tuple(map(float, poses[j]))) for j, i in enumerate(idx)]
+    full0 = cde.stats()["n_full_uploads"]
+    t_host, t_sync, patch_ms, patch_bytes = 0.0, 0.0, [], []
+    for j, i in enumerate(idx):
+        old = J.HazardEntity.placed_item(int(i), scene.items[i][1], int(i))
+        new = J.HazardEntity.placed_item(int(i), tuple(map(float, poses[j])), 100000 + int(i))
+        for change in (lambda: cde.deregister_hazard_by_entity(old),
+                       lambda: cde.register_hazard(J.Hazard(new, shapes[j], True))):
+            t0 = time.perf_counter()
+            change()
+            t1 = time.perf_counter()
+            cde.sync()
+            t2 = time.perf_counter()
+            t_host += t1 - t0
+            t_sync += t2 - t1
+            s = cde.stats()
+            patch_ms.append(s["patch_ms"])
+            patch_bytes.append(s["patch_bytes"])
+    full = cde.stats()["n_full_uploads"] - full0
+    k = 2 * n
+    out = {"changes": k, "quadtree_ms_per_change": t_host * 1e3 / k, "sync_ms_per_change": t_sync * 1e3 / k,
+           "patch_ms_median": float(np.median(patch_ms)), "patch_bytes_median": float(np.median(patch_bytes)),
+           "full_uploads": int(full), "full_upload_ms": st0["flatten_ms"] + st0["upload_ms"],
+           "image_bytes": st0["device_bytes"]}
+    log(f"update: {k} changes, quadtree {out['quadtree_ms_per_change']:.3f} ms + sync "
+        f"{out['sync_ms_per_change']:.3f} ms per change (median patch {out['patch_bytes_median'] / 1e3:.1f} kB); "
+        f"full upload {out['full_upload_ms']:.1f} ms; {full} full uploads")
+    return out
 
 
 def overlap_c4(stream):

@@ -171,13 +171,22 @@ void jg_snapshot_free(jg_snapshot* snap);
 jg_status jg_cde_bbox(const jg_cde* cde, float out[4]);
 jg_status jg_cde_hkey_exterior(const jg_cde* cde, jg_hazkey* out);
 
+/* Sizes of the device image: node records (the host node pool, free blocks included), entry
+ * and inline-edge records (arena use, including the slack of relocated ranges), slots and
+ * hazard vertices; device_bytes is the image's layout size with its headroom.  A full upload
+ * (flatten_ms, upload_ms) rebuilds the image in breadth-first order; a patch (patch_ms,
+ * patch_bytes) rewrites only the nodes and slots changed since the last upload. */
 typedef struct {
     uint64_t n_nodes, n_entries, n_edge_refs, n_hazard_vertices, n_slots;
     uint64_t device_bytes;
     double flatten_ms, upload_ms;
+    double patch_ms;
+    uint64_t patch_bytes, n_patches, n_full_uploads;
 } jg_cde_stats;
-/* Flattens the quadtree to its BFS structure-of-arrays form and uploads it to HBM when it
- * changed since the last upload (queries do this implicitly). */
+/* Brings the engine's HBM image up to date when it changed since the last upload (queries do
+ * this implicitly): in place, when the changes fit the image's headroom, else by a full
+ * rebuild.  Must not run while a launch that reads this engine is in flight on another
+ * thread; earlier launches are waited for. */
 jg_status jg_cde_sync(jg_cde* cde);
 jg_status jg_cde_get_stats(const jg_cde* cde, jg_cde_stats* out);
 /* The flattened tree in BFS order (same record layout as the oracle's tree dump):

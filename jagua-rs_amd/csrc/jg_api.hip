@@ -99,68 +99,115 @@ jg_status item_upload(jg_spolygon* it) {
     return JG_OK;
 }
 
+// Copies the patch segments of the staging buffer into the image, one workgroup per segment.
+__global__ __launch_bounds__(256) void k_scatter(uint8_t* image, const ImageSeg* segs, uint32_t n_segs,
+                                                 const uint8_t* data) {
+    for (uint32_t s = blockIdx.x; s < n_segs; s += gridDim.x) {
+        const ImageSeg g = segs[s];
+        const uint32_t* src = (const uint32_t*)(data + g.src);
+        uint32_t* dst = (uint32_t*)(image + g.dst);
+        const uint64_t words = g.bytes >> 2;
+        for (uint64_t w = threadIdx.x; w < words; w += blockDim.x) dst[w] = src[w];
+    }
+}
+
+static void set_view(jg_cde* c) {
+    const PoolImage& im = c->img;
+    uint8_t* b = (uint8_t*)c->dev.ptr;
+    CdeView& v = c->view;
+    v.nodes = (const DevNode*)(b + im.o_nd);
+    v.node_up = (const uint32_t*)(b + im.o_nu);
+    v.ent = (const DevEntry*)(b + im.o_en);
+    v.edges = (const float4*)(b + im.o_ed);
+    v.haz_bbox = (const float4*)(b + im.o_hb);
+    v.haz_poi = (const float4*)(b + im.o_hp);
+    v.haz_vr = (const uint2*)(b + im.o_hv);
+    v.haz_verts = (const float2*)(b + im.o_vv);
+    v.n_slots = (uint32_t)im.slot_ver.size();
+    v.exterior_slot = c->exterior.idx;
+    v.cd_threshold = c->cfg.cd_threshold;
+    v.n_nodes = (uint32_t)im.nodes.size();
+    c->stats.n_nodes = im.nodes.size();
+    c->stats.n_entries = im.ent.size();
+    c->stats.n_edge_refs = im.edges.size();
+    c->stats.n_hazard_vertices = im.verts.size();
+    c->stats.n_slots = im.slot_ver.size();
+    c->stats.device_bytes = im.total;
+}
+
+// Applies the changes since the last upload in place; false when a full upload is needed.
+static jg_status cde_patch(jg_cde* c, bool& done) {
+    done = false;
+    if (!c->dev_valid || !c->img.valid || c->dev.cap < c->img.total) return JG_OK;
+    auto t0 = std::chrono::steady_clock::now();
+    if (!image_patch(c->img, c->qt, c->map, c->staging, c->segs)) {
+        c->img.valid = false;
+        return JG_OK;
+    }
+    c->img.valid = false;                  // until the device holds the patched image
+    const uint64_t n_segs = c->segs.size();
+    const uint64_t seg_bytes = (sizeof(ImageSeg) * n_segs + 255) & ~(uint64_t)255;
+    const uint64_t data_bytes = c->staging.size();
+    for (const ImageSeg& g : c->segs)
+        if (g.dst + g.bytes > c->img.total || g.src + g.bytes > data_bytes || (g.dst | g.src | g.bytes) & 3)
+            return fail(JG_ERR_INTERNAL, "image patch segment out of range");
+    if (n_segs) {
+        jg_status s = c->staging_dev.reserve(seg_bytes + data_bytes);
+        if (s != JG_OK) return s;
+        HIP_TRY(hipDeviceSynchronize());   // earlier launches may still read the image
+        uint8_t* sd = (uint8_t*)c->staging_dev.ptr;
+        HIP_TRY(hipMemcpy(sd, c->segs.data(), sizeof(ImageSeg) * n_segs, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(sd + seg_bytes, c->staging.data(), data_bytes, hipMemcpyHostToDevice));
+        const uint32_t grid = (uint32_t)(n_segs < 8192 ? n_segs : 8192);
+        hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, 0, (uint8_t*)c->dev.ptr, (const ImageSeg*)sd,
+                           (uint32_t)n_segs, (const uint8_t*)(sd + seg_bytes));
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(0));
+    }
+    c->img.valid = true;
+    set_view(c);
+    auto t1 = std::chrono::steady_clock::now();
+    c->stats.patch_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    c->stats.patch_bytes = data_bytes;
+    c->stats.n_patches++;
+    done = true;
+    return JG_OK;
+}
+
 jg_status cde_upload(jg_cde* c) {
     if (!c->dirty && c->dev_valid) return JG_OK;
     jg_status s = device_ready();
     if (s != JG_OK) return s;
-    auto t0 = std::chrono::steady_clock::now();
-    FlatCDE f;
-    flatten(c->qt, c->map, f);
-    auto t1 = std::chrono::steady_clock::now();
-    size_t off = 0;
-    auto place = [&](size_t bytes) {
-        size_t o = off;
-        off = align256(off + bytes);
-        return o;
-    };
-    const size_t o_nd = place(sizeof(FlatNode) * f.nodes.size());
-    const size_t o_nu = place(sizeof(uint32_t) * f.node_up.size());
-    const size_t o_en = place(sizeof(FlatEntry) * f.ent.size());
-    const size_t o_ed = place(sizeof(float4) * (f.edges.size() + 1));
-    const size_t o_hb = place(sizeof(float4) * f.haz_bbox.size());
-    const size_t o_hp = place(sizeof(float4) * f.haz_poi.size());
-    const size_t o_hv = place(sizeof(uint2) * f.haz_vr.size());
-    const size_t o_vv = place(sizeof(float2) * (f.haz_verts.size() + 1));
-    const size_t total = off;
-    std::vector<uint8_t> host(total, 0);
-    auto put = [&](size_t o, const void* src, size_t bytes) {
-        if (bytes) std::memcpy(host.data() + o, src, bytes);
-    };
-    put(o_nd, f.nodes.data(), sizeof(FlatNode) * f.nodes.size());
-    put(o_nu, f.node_up.data(), sizeof(uint32_t) * f.node_up.size());
-    put(o_en, f.ent.data(), sizeof(FlatEntry) * f.ent.size());
-    put(o_ed, f.edges.data(), sizeof(float4) * f.edges.size());
-    put(o_hb, f.haz_bbox.data(), sizeof(float4) * f.haz_bbox.size());
-    put(o_hp, f.haz_poi.data(), sizeof(float4) * f.haz_poi.size());
-    put(o_hv, f.haz_vr.data(), sizeof(uint2) * f.haz_vr.size());
-    put(o_vv, f.haz_verts.data(), sizeof(float2) * f.haz_verts.size());
-    if (c->dev.ptr) HIP_TRY(hipDeviceSynchronize());   // earlier launches may still read the old image
-    s = c->dev.reserve(total);
+    bool done = false;
+    s = cde_patch(c, done);
     if (s != JG_OK) return s;
-    HIP_TRY(hipMemcpy(c->dev.ptr, host.data(), total, hipMemcpyHostToDevice));
-    auto t2 = std::chrono::steady_clock::now();
-    uint8_t* b = (uint8_t*)c->dev.ptr;
-    CdeView& v = c->view;
-    v.nodes = (const DevNode*)(b + o_nd);
-    v.node_up = (const uint32_t*)(b + o_nu);
-    v.ent = (const DevEntry*)(b + o_en);
-    v.edges = (const float4*)(b + o_ed);
-    v.haz_bbox = (const float4*)(b + o_hb);
-    v.haz_poi = (const float4*)(b + o_hp);
-    v.haz_vr = (const uint2*)(b + o_hv);
-    v.haz_verts = (const float2*)(b + o_vv);
-    v.n_slots = (uint32_t)f.haz_bbox.size();
-    v.exterior_slot = c->exterior.idx;
-    v.cd_threshold = c->cfg.cd_threshold;
-    v.n_nodes = (uint32_t)f.nodes.size();
-    c->stats.n_nodes = f.nodes.size();
-    c->stats.n_entries = f.ent.size();
-    c->stats.n_edge_refs = f.edges.size();
-    c->stats.n_hazard_vertices = f.haz_verts.size();
-    c->stats.n_slots = f.haz_bbox.size();
-    c->stats.device_bytes = total;
-    c->stats.flatten_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    c->stats.upload_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    if (!done) {
+        auto t0 = std::chrono::steady_clock::now();
+        PoolImage& im = c->img;
+        image_build(im, c->qt, c->map);
+        auto t1 = std::chrono::steady_clock::now();
+        std::vector<uint8_t> host(im.total, 0);
+        auto put = [&](uint64_t o, const void* src, uint64_t bytes) {
+            if (bytes) std::memcpy(host.data() + o, src, bytes);
+        };
+        put(im.o_nd, im.nodes.data(), sizeof(FlatNode) * im.nodes.size());
+        put(im.o_nu, im.up.data(), sizeof(uint32_t) * im.up.size());
+        put(im.o_en, im.ent.data(), sizeof(FlatEntry) * im.ent.size());
+        put(im.o_ed, im.edges.data(), sizeof(float4) * im.edges.size());
+        put(im.o_hb, im.hbb.data(), sizeof(float4) * im.hbb.size());
+        put(im.o_hp, im.hpoi.data(), sizeof(float4) * im.hpoi.size());
+        put(im.o_hv, im.hvr.data(), sizeof(uint2) * im.hvr.size());
+        put(im.o_vv, im.verts.data(), sizeof(float2) * im.verts.size());
+        if (c->dev.ptr) HIP_TRY(hipDeviceSynchronize());   // earlier launches may still read the old image
+        s = c->dev.reserve(im.total);
+        if (s != JG_OK) return s;
+        HIP_TRY(hipMemcpy(c->dev.ptr, host.data(), im.total, hipMemcpyHostToDevice));
+        auto t2 = std::chrono::steady_clock::now();
+        set_view(c);
+        c->stats.flatten_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        c->stats.upload_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+        c->stats.n_full_uploads++;
+    }
     c->dirty = false;
     c->dev_valid = true;
     c->generation++;
@@ -519,17 +566,6 @@ extern "C" jg_status jg_transforms_from_dt(const float* dt, uint64_t n, float* o
 }
 
 // ------------------------------------------------------------------------------ ABI: engine
-static QEntry root_entry(const Polygon& s, jg_hazkey k) {     // QTHazard::from_root (qt_hazard.rs:36-43)
-    QEntry e;
-    e.slot = k.idx;
-    e.version = k.version;
-    e.presence = PRES_PARTIAL;
-    e.ff = s.bbox;
-    e.edges.resize(s.n());
-    for (uint32_t i = 0; i < s.n(); i++) e.edges[i] = i;
-    return e;
-}
-
 static jg_hazkey cde_register(jg_cde* c, const jg_hazard_entity& ent, const Polygon& shape, bool dynamic) {
     HazardRec r;
     r.entity = ent;

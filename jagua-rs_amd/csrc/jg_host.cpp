@@ -364,6 +364,7 @@ void QuadTree::init(uint8_t depth, float4 bbox, uint8_t cdt) {
     pool[0].level = depth;
     pool[0].bbox = bbox;
     free_blocks.clear();
+    touched.assign(1, 0);
     cd_threshold = cdt;
 }
 
@@ -386,6 +387,8 @@ int32_t QuadTree::alloc_children(int32_t parent) {
         if (!(c.bbox.x < c.bbox.z && c.bbox.y < c.bbox.w)) c.bbox = c.bbox;  // degenerate boxes are kept as is
     }
     pool[parent].child = base;
+    touched.push_back(parent);
+    for (int k = 0; k < 4; k++) touched.push_back(base + k);
     return base;
 }
 
@@ -396,6 +399,37 @@ void QuadTree::free_subtree_children(int32_t node) {
     for (int k = 0; k < 4; k++) pool[base + k] = QNode{};
     free_blocks.push_back(base);
     pool[node].child = -1;
+    touched.push_back(node);
+}
+
+QEntry root_entry(const Polygon& s, jg_hazkey k) {     // QTHazard::from_root (qt_hazard.rs:36-43)
+    QEntry e;
+    e.slot = k.idx;
+    e.version = k.version;
+    e.presence = PRES_PARTIAL;
+    e.ff = s.bbox;
+    e.edges.resize(s.n());
+    for (uint32_t i = 0; i < s.n(); i++) e.edges[i] = i;
+    return e;
+}
+
+void QuadTree::compact_bfs() {
+    std::vector<int32_t> order{0};
+    for (size_t i = 0; i < order.size(); i++)
+        if (pool[order[i]].child >= 0)
+            for (int k = 0; k < 4; k++) order.push_back(pool[order[i]].child + k);
+    std::vector<QNode> np(order.size());
+    int32_t next_child = 1;
+    for (size_t i = 0; i < order.size(); i++) {
+        np[i] = std::move(pool[order[i]]);
+        if (np[i].child >= 0) {
+            np[i].child = next_child;
+            next_child += 4;
+        }
+    }
+    pool = std::move(np);
+    free_blocks.clear();
+    touched.clear();
 }
 
 static void add_entry(QNode& n, QEntry e) {                         // qt_hazard_vec.rs:24-43
@@ -499,6 +533,7 @@ void QuadTree::register_hazard(int32_t node, QEntry e, const SlotMap& map) {
     }
     if (pool[node].child >= 0) push_to_children(node, e, map);
     add_entry(pool[node], std::move(e));
+    touched.push_back(node);
 }
 
 // QTNode::deregister_hazard (qt_node.rs:83-94)
@@ -509,6 +544,7 @@ void QuadTree::deregister_hazard(int32_t node, uint32_t slot, uint32_t version) 
     if (i == n.haz.size()) return;
     n.n_active_edges -= n.haz[i].presence == PRES_PARTIAL ? n.haz[i].edges.size() : 0;
     n.haz.erase(n.haz.begin() + (long)i);
+    touched.push_back(node);
     bool any_partial = false;
     for (const QEntry& h : n.haz) any_partial |= h.presence == PRES_PARTIAL;
     if (!any_partial) {
@@ -579,6 +615,183 @@ void flatten(const QuadTree& qt, const SlotMap& map, FlatCDE& f) {
         f.haz_vr[s] = make_uint2((uint32_t)f.haz_verts.size(), r.shape.n());
         f.haz_verts.insert(f.haz_verts.end(), r.shape.v.begin(), r.shape.v.end());
     }
+}
+
+// ---------------------------------------------------------------------------------------
+// pool-ordered device image with in-place updates
+// ---------------------------------------------------------------------------------------
+static void image_write_node(PoolImage& im, const QuadTree& qt, const SlotMap& map, uint32_t i) {
+    const QNode& n = qt.pool[i];
+    const uint32_t count = (uint32_t)n.haz.size();
+    // a full build packs the ranges tightly (the layout queries run on); a range that has to
+    // grow later moves to the arena's end with room for half as much again
+    if (count > im.ent_cap[i]) {
+        im.garbage += im.ent_cap[i];
+        im.ent_off[i] = (uint32_t)im.ent.size();
+        im.ent_cap[i] = im.building ? count : count + count / 2 + 1;
+        im.ent.resize(im.ent.size() + im.ent_cap[i]);
+    }
+    const bool tests_edges = n.child < 0 || n.n_active_edges <= qt.cd_threshold;
+    uint32_t need = 0;
+    if (tests_edges)
+        for (const QEntry& h : n.haz)
+            if (h.presence == PRES_PARTIAL) need += (uint32_t)h.edges.size();
+    if (need > im.edge_cap[i]) {
+        im.garbage += im.edge_cap[i];
+        im.edge_off[i] = (uint32_t)im.edges.size();
+        im.edge_cap[i] = im.building ? need : need + need / 2 + 2;
+        im.edges.resize(im.edges.size() + im.edge_cap[i]);
+    }
+    FlatNode fn{};
+    fn.bbox = n.bbox;
+    fn.first_child = n.child;
+    fn.haz_begin = im.ent_off[i];
+    fn.haz_info = count | (count && n.haz[0].presence == PRES_ENTIRE ? 0x80000000u : 0u);
+    fn.n_active_edges = (uint32_t)std::min<uint64_t>(n.n_active_edges, 0xffffffffu);
+    uint32_t ec = im.edge_off[i];
+    for (uint32_t k = 0; k < count; k++) {
+        const QEntry& h = n.haz[k];
+        FlatEntry fe{};
+        fe.slot_flags = h.slot | (h.presence == PRES_ENTIRE ? 0x80000000u : 0u);
+        fe.n_edges_total = h.presence == PRES_PARTIAL ? (uint32_t)h.edges.size() : 0u;
+        if (h.presence == PRES_PARTIAL) {
+            fe.ff = h.ff;
+            if (tests_edges) {
+                fe.edge_begin = ec;
+                fe.edge_count = (uint32_t)h.edges.size();
+                const Polygon& sh = map.slots[h.slot].rec.shape;
+                for (uint32_t ei : h.edges) im.edges[ec++] = sh.edge(ei);
+            }
+        }
+        im.ent[im.ent_off[i] + k] = fe;
+    }
+    im.nodes[i] = fn;
+    if (n.child >= 0)
+        for (uint32_t k = 0; k < 4; k++) im.up[(uint32_t)n.child + k] = (i << 2) | k;
+}
+
+static void image_write_slot(PoolImage& im, const SlotMap& map, uint32_t s) {
+    const HazardRec& r = map.slots[s].rec;
+    im.hbb[s] = r.shape.bbox;
+    im.hpoi[s] = make_float4(r.shape.poi.x, r.shape.poi.y, r.shape.poi.z, r.interior() ? 1.0f : 0.0f);
+    im.hvr[s] = make_uint2((uint32_t)im.verts.size(), r.shape.n());
+    im.verts.insert(im.verts.end(), r.shape.v.begin(), r.shape.v.end());
+    im.slot_ver[s] = map.slots[s].version;
+}
+
+static uint64_t with_headroom(uint64_t n) { return n + n / 2 + 1024; }
+
+void image_build(PoolImage& im, QuadTree& qt, const SlotMap& map) {
+    qt.compact_bfs();
+    const uint32_t N = (uint32_t)qt.pool.size();
+    im = PoolImage{};
+    im.nodes.assign(N, FlatNode{});
+    im.up.assign(N, 0u);
+    im.ent_off.assign(N, 0u);
+    im.ent_cap.assign(N, 0u);
+    im.edge_off.assign(N, 0u);
+    im.edge_cap.assign(N, 0u);
+    im.building = true;
+    for (uint32_t i = 0; i < N; i++) image_write_node(im, qt, map, i);
+    im.building = false;
+    const uint32_t S = (uint32_t)map.slots.size();
+    im.hbb.assign(S, make_float4(0, 0, 0, 0));
+    im.hpoi.assign(S, make_float4(0, 0, 0, 0));
+    im.hvr.assign(S, make_uint2(0, 0));
+    im.slot_ver.assign(S, 0u);
+    for (uint32_t s = 1; s < S; s++)
+        if (map.slots[s].occ) image_write_slot(im, map, s);
+    im.cap_nodes = with_headroom(N);
+    im.cap_ent = with_headroom(im.ent.size());
+    im.cap_edges = with_headroom(im.edges.size());
+    im.cap_slots = with_headroom(S);
+    im.cap_verts = with_headroom(im.verts.size());
+    uint64_t off = 0;
+    auto place = [&](uint64_t bytes) {
+        const uint64_t o = off;
+        off = (off + bytes + 255) & ~(uint64_t)255;
+        return o;
+    };
+    im.o_nd = place(sizeof(FlatNode) * im.cap_nodes);
+    im.o_nu = place(sizeof(uint32_t) * im.cap_nodes);
+    im.o_en = place(sizeof(FlatEntry) * im.cap_ent);
+    im.o_ed = place(sizeof(float4) * im.cap_edges);
+    im.o_hb = place(sizeof(float4) * im.cap_slots);
+    im.o_hp = place(sizeof(float4) * im.cap_slots);
+    im.o_hv = place(sizeof(uint2) * im.cap_slots);
+    im.o_vv = place(sizeof(float2) * im.cap_verts);
+    im.total = off;
+    im.valid = true;
+    qt.touched.clear();
+}
+
+bool image_patch(PoolImage& im, QuadTree& qt, const SlotMap& map, std::vector<uint8_t>& staging,
+                 std::vector<ImageSeg>& segs) {
+    staging.clear();
+    segs.clear();
+    const uint32_t N = (uint32_t)qt.pool.size();
+    if (!im.valid || N > im.cap_nodes || map.slots.size() > im.cap_slots) return false;
+    if (N > im.nodes.size()) {
+        im.nodes.resize(N, FlatNode{});
+        im.up.resize(N, 0u);
+        im.ent_off.resize(N, 0u);
+        im.ent_cap.resize(N, 0u);
+        im.edge_off.resize(N, 0u);
+        im.edge_cap.resize(N, 0u);
+    }
+    std::vector<uint32_t> nodes;
+    {
+        std::vector<char> seen(N, 0);
+        for (int32_t t : qt.touched)
+            if (t >= 0 && (uint32_t)t < N && !seen[t]) {
+                seen[t] = 1;
+                nodes.push_back((uint32_t)t);
+            }
+    }
+    for (uint32_t i : nodes) image_write_node(im, qt, map, i);
+    const uint32_t S = (uint32_t)map.slots.size();
+    if (im.slot_ver.size() < S) {
+        im.hbb.resize(S, make_float4(0, 0, 0, 0));
+        im.hpoi.resize(S, make_float4(0, 0, 0, 0));
+        im.hvr.resize(S, make_uint2(0, 0));
+        im.slot_ver.resize(S, 0u);
+    }
+    std::vector<uint32_t> slots;
+    const size_t verts0 = im.verts.size();
+    for (uint32_t s = 1; s < S; s++)
+        if (map.slots[s].occ && im.slot_ver[s] != map.slots[s].version) {
+            image_write_slot(im, map, s);
+            slots.push_back(s);
+        }
+    if (im.ent.size() > im.cap_ent || im.edges.size() > im.cap_edges || im.verts.size() > im.cap_verts) return false;
+    if (im.garbage > (uint64_t)(im.ent.size() + im.edges.size()) / 2 + 4096) return false;
+    auto add = [&](uint64_t dst, const void* src, uint64_t bytes) {
+        if (!bytes) return;
+        segs.push_back(ImageSeg{dst, staging.size(), bytes});
+        const uint8_t* p = (const uint8_t*)src;
+        staging.insert(staging.end(), p, p + bytes);
+    };
+    for (uint32_t i : nodes) {
+        add(im.o_nd + sizeof(FlatNode) * (uint64_t)i, &im.nodes[i], sizeof(FlatNode));
+        const uint32_t cnt = im.nodes[i].haz_info & 0x3fffffffu;
+        if (cnt) add(im.o_en + sizeof(FlatEntry) * (uint64_t)im.ent_off[i], &im.ent[im.ent_off[i]], sizeof(FlatEntry) * (uint64_t)cnt);
+        uint32_t ne = 0;
+        for (uint32_t k = 0; k < cnt; k++) ne += im.ent[im.ent_off[i] + k].edge_count;
+        if (ne) add(im.o_ed + sizeof(float4) * (uint64_t)im.edge_off[i], &im.edges[im.edge_off[i]], sizeof(float4) * (uint64_t)ne);
+        if (im.nodes[i].first_child >= 0) {
+            const uint32_t c = (uint32_t)im.nodes[i].first_child;
+            add(im.o_nu + sizeof(uint32_t) * (uint64_t)c, &im.up[c], sizeof(uint32_t) * 4);
+        }
+    }
+    for (uint32_t s : slots) {
+        add(im.o_hb + sizeof(float4) * (uint64_t)s, &im.hbb[s], sizeof(float4));
+        add(im.o_hp + sizeof(float4) * (uint64_t)s, &im.hpoi[s], sizeof(float4));
+        add(im.o_hv + sizeof(uint2) * (uint64_t)s, &im.hvr[s], sizeof(uint2));
+    }
+    if (im.verts.size() > verts0)
+        add(im.o_vv + sizeof(float2) * (uint64_t)verts0, im.verts.data() + verts0, sizeof(float2) * (im.verts.size() - verts0));
+    qt.touched.clear();
+    return true;
 }
 
 }  // namespace jg

@@ -87,10 +87,13 @@ struct QNode {                           // QTNode (qt_node.rs:14-25)
 struct QuadTree {
     std::vector<QNode> pool;
     std::vector<int32_t> free_blocks;
+    std::vector<int32_t> touched;        // nodes whose record, entries or children changed since the last image patch
     uint8_t cd_threshold = 16;
     void init(uint8_t depth, float4 bbox, uint8_t cdt);
     void register_hazard(int32_t node, QEntry e, const SlotMap& map);
     void deregister_hazard(int32_t node, uint32_t slot, uint32_t version);
+    // Renumbers the pool in breadth-first order from the root and drops free blocks.
+    void compact_bfs();
   private:
     int32_t alloc_children(int32_t parent);
     void free_subtree_children(int32_t node);
@@ -127,5 +130,42 @@ struct FlatCDE {
 };
 
 void flatten(const QuadTree& qt, const SlotMap& map, FlatCDE& out);
+
+// The entry a new hazard registers at the root with: Partial, every edge, ff = its bbox.
+QEntry root_entry(const Polygon& s, jg_hazkey k);
+
+// The device image in pool order: node i of the image is node i of the host pool, whose
+// children already come in blocks of 4 consecutive nodes.  Every node owns a range of entry
+// records and one of inline edges; after an update only the touched nodes are rewritten, in
+// place while their ranges are large enough, else at the end of the arena.  The image is
+// rebuilt whole when a device array outgrows its capacity or garbage exceeds the live data.
+struct ImageSeg {                         // one copy of the staging buffer into the device image
+    uint64_t dst, src, bytes;             // byte offsets and size (multiples of 4)
+};
+
+struct PoolImage {
+    bool valid = false;
+    std::vector<FlatNode> nodes;
+    std::vector<uint32_t> up;
+    std::vector<uint32_t> ent_off, ent_cap, edge_off, edge_cap;
+    std::vector<FlatEntry> ent;
+    std::vector<float4> edges;
+    std::vector<float4> hbb, hpoi;
+    std::vector<uint2> hvr;
+    std::vector<float2> verts;
+    std::vector<uint32_t> slot_ver;       // version written per slot, 0 = none
+    uint64_t garbage = 0;                 // entry + edge records no node refers to
+    bool building = false;                // inside image_build: ranges are packed tightly
+    // device layout: record capacities and byte offsets
+    uint64_t cap_nodes = 0, cap_ent = 0, cap_edges = 0, cap_slots = 0, cap_verts = 0;
+    uint64_t o_nd = 0, o_nu = 0, o_en = 0, o_ed = 0, o_hb = 0, o_hp = 0, o_hv = 0, o_vv = 0, total = 0;
+};
+
+// Full build of the host mirror and the device layout (capacities with headroom).
+void image_build(PoolImage& im, QuadTree& qt, const SlotMap& map);
+// Rewrites the nodes and slots changed since the last build or patch into the host mirror and
+// lists the device copies; false when a full build is needed instead.
+bool image_patch(PoolImage& im, QuadTree& qt, const SlotMap& map, std::vector<uint8_t>& staging,
+                 std::vector<ImageSeg>& segs);
 
 }  // namespace jg

@@ -39,7 +39,11 @@ struct jg_cde {
     jg_hazkey exterior{0, 0};
     bool dirty = true;
     bool dev_valid = false;
-    jg::DevBlob dev;                   // flattened BFS image
+    jg::DevBlob dev;                   // the device image (pool order, with headroom)
+    jg::PoolImage img;                 // its host mirror
+    jg::DevBlob staging_dev;           // patch segments and their bytes
+    std::vector<uint8_t> staging;
+    std::vector<jg::ImageSeg> segs;
     jg::DevBlob filter_dev;            // JG_FILTER_KEYS mask
     jg::CdeView view{};
     jg_cde_stats stats{};

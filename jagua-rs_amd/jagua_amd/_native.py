@@ -47,7 +47,8 @@ class SPolygonInfoC(C.Structure):
 class CDEStatsC(C.Structure):
     _fields_ = [("n_nodes", C.c_uint64), ("n_entries", C.c_uint64), ("n_edge_refs", C.c_uint64),
                 ("n_hazard_vertices", C.c_uint64), ("n_slots", C.c_uint64), ("device_bytes", C.c_uint64),
-                ("flatten_ms", C.c_double), ("upload_ms", C.c_double)]
+                ("flatten_ms", C.c_double), ("upload_ms", C.c_double), ("patch_ms", C.c_double),
+                ("patch_bytes", C.c_uint64), ("n_patches", C.c_uint64), ("n_full_uploads", C.c_uint64)]
 
 
 class QuerySetC(C.Structure):

@@ -148,3 +148,51 @@ def test_collect_sets_beyond_the_slot_list(cuda):
     dts = S.transforms(22, 48, (500, 400, 3500, 1100), 0)
     st, ref = run_both(cde, oc, item, oitem, J.compose(dts), cap=300, cuda=cuda)
     assert ref["count"].max() > 256 and ref["count"].min() < 256
+
+
+@pytest.mark.parametrize("depth", [3, 8])
+def test_random_update_sequences(cuda, depth):
+    """Random batches of deregister / register / save-restore (lbf/tests/tests.rs:48-66 in a
+    loop).  Each batch reaches the device image as an in-place patch of the touched nodes and
+    slots; after every batch the plain detect / collect and a batched plan agree with the
+    oracle bit for bit, and the keys handed out agree with the oracle's slot map."""
+    sc = S.config1_scene(40)
+    sc.depth = depth
+    cde, keys, oc = engines(sc)
+    polys, dts = S.config1_queries(40, n=2048)
+    (item,), (oitem,) = items(polys)
+    mats = J.compose(dts)
+    m = torch.from_numpy(mats).to(cuda)
+    plan = J.QueryPlan([(cde, item, 0, len(mats), None)])
+    out = torch.empty(len(mats), dtype=torch.uint8, device=cuda)
+    run_both(cde, oc, item, oitem, mats, cuda=cuda)
+    full0 = cde.stats()["n_full_uploads"]
+    extra = [it for s in (41, 42, 43) for it in S.config1_scene(s).items]
+    live = [(k, ok, i, dt) for i, (k, ok, (_, dt)) in enumerate(zip(keys, oc.keys, sc.items))]
+    rng = np.random.default_rng(depth)
+    snap = None
+    for rnd in range(10):
+        for _ in range(int(rng.integers(1, 6))):
+            if live and rng.random() < 0.5:
+                k, ok, _, _ = live.pop(int(rng.integers(len(live))))
+                cde.deregister_hazard_by_key(k)
+                oc.deregister_key(ok)
+            elif extra:
+                loc, dt = extra.pop()
+                pk = 1000 + len(extra)
+                k = cde.register_hazard(J.Hazard(J.HazardEntity.placed_item(pk, dt, pk),
+                                                 J.SPolygon.new(loc).transform_clone(dt), True))
+                ok = oc.register_item(pk, loc, dt, pk)
+                assert (k.idx, k.version) == tuple(ok)
+                live.append((k, ok, pk, dt))
+        if rnd == 3:
+            snap, osnap = cde.save(), oc.save()
+        if rnd == 7:
+            cde.restore(snap)
+            oc.restore(osnap)
+            live = []          # keys after a restore are not tracked; later rounds only register
+        st, ref = run_both(cde, oc, item, oitem, mats, cuda=cuda)
+        plan.detect(m, out)
+        np.testing.assert_array_equal(out.cpu().numpy(), ref["status"])
+    s = cde.stats()
+    assert s["n_patches"] >= 8 and s["n_full_uploads"] - full0 <= 2, s

@@ -187,3 +187,22 @@ def test_workload_generators_deterministic():
     sc = S.config5_scene(seed=6, n_items=60, width=600.0, height=200.0, n_holes=2)
     assert sc.depth == 10 and all(64 <= len(v) <= 1024 for v, _ in sc.items)
     assert all(8 <= len(h) <= 64 for h in sc.holes)
+
+
+@pytest.mark.parametrize("seed,depth", [(1, 6), (2, 10), (3, 0), (4, 3)])
+def test_image_patches_equal_fresh_flatten(tmp_path, seed, depth):
+    """The device image after in-place patches (register / deregister batches applied as the
+    patch's copy segments to a byte buffer standing in for HBM) reads exactly like a fresh
+    breadth-first flatten of the host tree: nodes, up links, entries, inline edges, slots."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = tmp_path / "image_patch_check"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__",
+                    "-I/opt/rocm/include", os.path.join(root, "tests", "image_patch_check.cpp"),
+                    os.path.join(root, "jagua-rs_amd", "csrc", "jg_host.cpp"), "-o", str(exe)],
+                   check=True, capture_output=True)
+    out = subprocess.run([str(exe), str(seed), "150", str(depth)], capture_output=True, text=True)
+    assert out.returncode == 0, out.stderr
+    ok, patches, builds = out.stdout.split()
+    assert ok == "ok" and int(patches) > 100
