@@ -161,6 +161,7 @@ def run_c2(args):
         result["overlap"] = overlap_c4(stream)
         result["search"] = search_c2(cde, items, stream)
         result["update"] = update_c2(cde, scene, st)
+        result["surrogates"] = surrogates_piers()
 
     roof, cpu = None, None
     if not args.no_cpu:
@@ -251,6 +252,33 @@ def update_c2(cde, scene, st0, n=100):
     log(f"update: {k} changes, quadtree {out['quadtree_ms_per_change']:.3f} ms + sync "
         f"{out['sync_ms_per_change']:.3f} ms per change (median patch {out['patch_bytes_median'] / 1e3:.1f} kB); "
         f"full upload {out['full_upload_ms']:.1f} ms; {full} full uploads")
+    return out
+
+
+def surrogates_piers(n=512, n_sample=4):
+    """SPolygon::new + generate_surrogate with 2 piers (sp_surrogate.rs:33-53, piers.rs:25-98)
+    for n star polygons of 8-256 vertices: one batch with pole and pier searches on the GPU,
+    against the host routine (one thread) timed on the first n_sample polygons."""
+    import jagua_amd as J
+    from jagua_amd import synth as S
+    ps = [S.polygon(50_000 + i, 8, 256, 5.0, 30.0, i % 3) for i in range(n)]
+    cfg = J.SPSurrogateConfig(n_ff_poles=2, n_ff_piers=2)
+    cfg0 = J.SPSurrogateConfig(n_ff_poles=2, n_ff_piers=0)
+    J.SPolygon.new_batch(ps[:4], cfg)
+    t0 = time.perf_counter()
+    got, n_host = J.SPolygon.new_batch(ps, cfg)
+    t1 = time.perf_counter()
+    J.SPolygon.new_batch(ps, cfg0)
+    t2 = time.perf_counter()
+    ref = [J.SPolygon.new(p).generate_surrogate(cfg) for p in ps[:n_sample]]
+    t3 = time.perf_counter()
+    same = all((g.piers.view("u4") == r.piers.view("u4")).all() for g, r in zip(got, ref))
+    out = {"polygons": n, "n_ff_piers": 2, "batch_s": t1 - t0, "batch_without_piers_s": t2 - t1,
+           "host_s_per_polygon": (t3 - t2) / n_sample, "host_sample": n_sample, "host_completed": int(n_host),
+           "sample_bit_exact": bool(same)}
+    log(f"surrogates with piers: {n} polygons in {out['batch_s']:.3f} s on the GPU "
+        f"({out['batch_without_piers_s']:.3f} s without piers); host routine {out['host_s_per_polygon']:.3f} s "
+        f"per polygon; sample bit-exact: {same}")
     return out
 
 

@@ -123,14 +123,17 @@ jg_status jg_spolygon_transform_clone(const jg_spolygon* poly, const float dt[3]
 
 /* Many SPolygon::new (simple_polygon.rs:41-71) at once, each followed by
  * generate_surrogate (sp_surrogate.rs:33-53) when cfg is not NULL, with the pole searches
- * (fail_fast/pole.rs:12-80) run on the GPU: out[i] is bit-identical to jg_spolygon_new on
+ * (fail_fast/pole.rs:12-80) and, when cfg->n_ff_piers > 0, the pier searches
+ * (fail_fast/piers.rs:25-98) run on the GPU: out[i] is bit-identical to jg_spolygon_new on
  * xy[2*offsets[i] .. 2*offsets[i+1]) followed by jg_spolygon_generate_surrogate.  offsets
  * has n_polys + 1 entries.  A polygon whose search outgrows the device buffers (more than
- * 8192 cells on one level, or more poles than the largest pole limit) is searched by the
- * host routine instead; their number goes to *n_host_completed (may be NULL).  On error no
- * handle is returned and the message names the polygon. */
+ * 8192 cells on one level, more poles than the largest pole limit, a pier ray crossing its
+ * boundary more than 32 times) or that the reference would reject during pier generation is
+ * finished by the host routine instead; their number goes to *n_host_completed (may be
+ * NULL).  On error no handle is returned and the message names the polygon. */
 jg_status jg_spolygon_new_batch(const float* xy, const uint32_t* offsets, uint32_t n_polys,
-                                const jg_surrogate_config* cfg, jg_spolygon** out, uint32_t* n_host_completed);void jg_spolygon_free(jg_spolygon* poly);
+                                const jg_surrogate_config* cfg, jg_spolygon** out, uint32_t* n_host_completed);
+void jg_spolygon_free(jg_spolygon* poly);
 
 typedef struct {
     uint32_t n_vertices;

@@ -37,7 +37,7 @@ DEVICE = [
     "-munsafe-fp-atomics",
 ]
 
-SOURCES = ["jg_host.cpp", "jg_api.hip", "jg_batched.hip", "jg_overlap.hip", "jg_poles.hip", "jg_search.hip"]
+SOURCES = ["jg_host.cpp", "jg_api.hip", "jg_batched.hip", "jg_overlap.hip", "jg_poles.hip", "jg_search.hip", "jg_piers.hip"]
 
 
 def _run(cmd):

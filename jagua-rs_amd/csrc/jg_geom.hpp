@@ -258,6 +258,85 @@ JG_HD float pole_cell(const float2* v, uint32_t n, float4 bb, const float4* pole
     return inside ? d : -d;
 }
 
+// ---- generate_piers (fail_fast/piers.rs), release-build constants (:16-20)
+constexpr uint32_t PIER_RAYS = 200, PIER_ANGLES = 90, PIER_GRID = 100;
+constexpr uint32_t PIER_N_RAYS = PIER_RAYS * PIER_ANGLES, PIER_N_GRID = PIER_GRID * PIER_GRID;
+constexpr float PIER_TRIM = 0.999f, PIER_INFL = 0.10f;
+
+// ndarray 0.17 Array::linspace element i: a + step * i with step = (b - a) / (n - 1)
+JG_HD float linspace_at(float a, float b, uint32_t n, uint32_t i) {
+    const float step = n > 1 ? (b - a) / (float)(n - 1) : 0.0f;
+    return a + step * (float)i;
+}
+
+// Ray i of the angle with (sin, cos) = (s, c) (generate_ray_transformations, piers.rs:100-123):
+// the base ray r0-r1 through the centroid, translated by (tx, 0) then rotated,
+// rot_m(angle) . transl_m(tx, 0) multiplied out term by term (transformation.rs:156-178).
+JG_HD float4 pier_ray(float s, float c, float tx, float2 r0, float2 r1) {
+    const Mat m{c * 1.0f + -s * 0.0f + 0.0f * 0.0f, c * 0.0f + -s * 1.0f + 0.0f * 0.0f,
+                c * tx + -s * 0.0f + 0.0f * 1.0f,  s * 1.0f + c * 0.0f + 0.0f * 0.0f,
+                s * 0.0f + c * 1.0f + 0.0f * 0.0f, s * tx + c * 0.0f + 0.0f * 1.0f};
+    const float2 a = xform(m, r0.x, r0.y), b = xform(m, r1.x, r1.y);
+    return make_float4(a.x, a.y, b.x, b.y);
+}
+
+// Edge::collides_at (edge.rs:179-228) of polygon edge e with ray r, once edge_edge(e, r)
+// holds: the intersection point and its distance to the ray's start (clip, piers.rs:128-133).
+JG_HD float2 pier_hit(float4 e, float4 r, float& dist) {
+    const float x1 = e.x, y1 = e.y, x2 = e.z, y2 = e.w, x3 = r.x, y3 = r.y, x4 = r.z, y4 = r.w;
+    const float t = ((x2 - x4) * (y4 - y3) - (y2 - y4) * (x4 - x3)) / ((x2 - x1) * (y4 - y3) - (y2 - y1) * (x4 - x3));
+    const float2 loc = make_float2(x2 + t * (x1 - x2), y2 + t * (y1 - y2));
+    const float dx = r.x - loc.x, dy = r.y - loc.y;
+    dist = sqrtf(dx * dx + dy * dy);
+    return loc;
+}
+
+// Edge::scale(CLIPPING_TRIM) of the clip from st to en (edge.rs:39-46)
+JG_HD float4 pier_trim(float2 st, float2 en) {
+    const float ddx = en.x - st.x, ddy = en.y - st.y;
+    return make_float4(st.x - ddx * (PIER_TRIM - 1.0f) / 2.0f, st.y - ddy * (PIER_TRIM - 1.0f) / 2.0f,
+                       en.x + ddx * (PIER_TRIM - 1.0f) / 2.0f, en.y + ddy * (PIER_TRIM - 1.0f) / 2.0f);
+}
+
+// Per-polygon constants of generate_piers: the inflated square bbox, the base ray through the
+// centroid (piers.rs:34-41), the ray spacing, the ray influence radius (:63) and the forfeit
+// distance (:64).
+struct PierSetup {
+    float4 sq;
+    float2 r0, r1;
+    float dx, infl, forfeit, pad;
+};
+
+// Circle::distance_to(Point) (circle.rs:136-147)
+JG_HD float circle_point_dist(float4 c, float px, float py) {
+    const float ddx = px - c.x, ddy = py - c.y;
+    const float sqd = ddx * ddx + ddy * ddy;
+    return sqd < c.z * c.z ? 0.0f : sqrtf(sqd) - c.z;
+}
+
+// Grid point (i, j) of generate_unrepresented_point_grid (piers.rs:153-176): kept when it lies
+// in the shape and outside every pole; d_pole = min_distances_to_poles (:227-237).
+JG_HD bool pier_grid_point(const float2* v, uint32_t n, float4 bb, const float4* poles, uint32_t n_poles,
+                           const PierSetup& ps, uint32_t i, uint32_t j, float2& p, float& d_pole) {
+    p = make_float2(linspace_at(ps.sq.x, ps.sq.z, PIER_GRID, i), linspace_at(ps.sq.y, ps.sq.w, PIER_GRID, j));
+    if (!poly_has_point(v, n, bb, p.x, p.y)) return false;
+    for (uint32_t k = 0; k < n_poles; k++) {
+        const float dx = p.x - poles[k].x, dy = p.y - poles[k].y;
+        if (dx * dx + dy * dy <= poles[k].z * poles[k].z) return false;
+    }
+    float d = ps.forfeit;
+    for (uint32_t k = 0; k < n_poles; k++) d = mn(d, circle_point_dist(poles[k], p.x, p.y));
+    d_pole = d;
+    return true;
+}
+
+// The score of grid point g for a candidate ray at distance dn (loss_function, piers.rs:185-213)
+JG_HD float pier_point_loss(float d_ray, float d_pole, float dn, float infl) {
+    const float d = mn(d_ray, dn);
+    const float v = d < infl ? mn(d_pole, d) : d_pole;
+    return v * v;
+}
+
 // Hashed per-query hazard filter and set hash (specified in include/jagua_amd.h).
 JG_HD uint32_t fmix32(uint32_t h) {
     h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;

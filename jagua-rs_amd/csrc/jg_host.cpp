@@ -170,10 +170,37 @@ static bool surrogate_poles(const Polygon& shape, const jg_surrogate_config& cfg
     }
 }
 
-// ndarray 0.17 linspace: a + step * i with step = (b - a) / (n - 1)
-static float lin(float a, float b, size_t n, size_t i) {
-    const float step = n > 1 ? (b - a) / (float)(n - 1) : 0.0f;
-    return a + step * (float)i;
+// SPolygon::centroid (simple_polygon.rs:173-192)
+static float2 polygon_centroid(const Polygon& shape) {
+    float gx = 0.0f, gy = 0.0f;
+    for (uint32_t i = 0; i < shape.n(); i++) {
+        const float2 a = shape.v[i], b = shape.v[i + 1 == shape.n() ? 0 : i + 1];
+        gx += (a.x + b.x) * (a.x * b.y - b.x * a.y);
+        gy += (a.y + b.y) * (a.x * b.y - b.x * a.y);
+    }
+    return make_float2(gx / (6.0f * shape.area), gy / (6.0f * shape.area));
+}
+
+void pier_setup(const Polygon& shape, PierSetup& o) {
+    const float4 bb = shape.bbox;
+    o.sq = box_inflate_to_square(bb);
+    const float2 g = polygon_centroid(shape);
+    const float sq_h = o.sq.w - o.sq.y;
+    o.r0 = make_float2(g.x, g.y - 2.0f * sq_h);
+    o.r1 = make_float2(g.x, g.y + 2.0f * sq_h);
+    o.dx = (o.sq.z - o.sq.x) / (float)PIER_RAYS;
+    o.infl = PIER_INFL * (o.sq.z - o.sq.x);
+    const float bw = bb.z - bb.x, bh = bb.w - bb.y;
+    o.forfeit = sqrtf((bw * bw) * (bh * bh));
+}
+
+void pier_angles(float* sc) {
+    const float PI_F = 3.14159265358979323846f;
+    for (uint32_t ai = 0; ai < PIER_ANGLES; ai++) {
+        const float ang = linspace_at(0.0f, PI_F, PIER_ANGLES + 1, ai);
+        sc[2 * ai] = sinf(ang);
+        sc[2 * ai + 1] = cosf(ang);
+    }
 }
 
 // generate_piers (fail_fast/piers.rs:25-226), release-build constants (:16-20)
@@ -181,97 +208,59 @@ static bool surrogate_piers(const Polygon& shape, uint32_t n, const std::vector<
                             std::vector<float4>& out) {
     out.clear();
     if (n == 0) return true;
-    const size_t RAYS = 200, ANGLES = 90, GRID = 100;
-    const float TRIM = 0.999f, INFL = 0.10f, PI_F = 3.14159265358979323846f;
-    const float4 bb = shape.bbox;
-    const float4 sq = box_inflate_to_square(bb);
-    // SPolygon::centroid (simple_polygon.rs:173-192)
-    float gx = 0.0f, gy = 0.0f;
-    for (uint32_t i = 0; i < shape.n(); i++) {
-        const float2 a = shape.v[i], b = shape.v[i + 1 == shape.n() ? 0 : i + 1];
-        gx += (a.x + b.x) * (a.x * b.y - b.x * a.y);
-        gy += (a.y + b.y) * (a.x * b.y - b.x * a.y);
-    }
-    gx /= 6.0f * shape.area;
-    gy /= 6.0f * shape.area;
-    const float sq_h = sq.w - sq.y;
-    const float2 r0 = make_float2(gx, gy - 2.0f * sq_h), r1 = make_float2(gx, gy + 2.0f * sq_h);
-    const float dx = (sq.z - sq.x) / (float)RAYS;
+    PierSetup ps;
+    pier_setup(shape, ps);
+    float sc[2 * PIER_ANGLES];
+    pier_angles(sc);
     std::vector<float4> clipped;
-    for (size_t ai = 0; ai < ANGLES; ai++) {
-        const float ang = lin(0.0f, PI_F, ANGLES + 1, ai);
-        const float s = sinf(ang), c = cosf(ang);
-        for (size_t i = 0; i < RAYS; i++) {
-            const float tx = sq.x + dx * (float)i;
-            // rot_m(angle) . transl_m(tx, 0) (transformation.rs:156-178), term by term
-            const float m00 = c * 1.0f + -s * 0.0f + 0.0f * 0.0f, m01 = c * 0.0f + -s * 1.0f + 0.0f * 0.0f;
-            const float m02 = c * tx + -s * 0.0f + 0.0f * 1.0f;
-            const float m10 = s * 1.0f + c * 0.0f + 0.0f * 0.0f, m11 = s * 0.0f + c * 1.0f + 0.0f * 0.0f;
-            const float m12 = s * tx + c * 0.0f + 0.0f * 1.0f;
-            const Mat m{m00, m01, m02, m10, m11, m12};
-            const float2 a = xform(m, r0.x, r0.y), b = xform(m, r1.x, r1.y);
-            if (polygon_has_point(shape, a.x, a.y) || polygon_has_point(shape, b.x, b.y)) return false;
+    for (uint32_t ai = 0; ai < PIER_ANGLES; ai++)
+        for (uint32_t i = 0; i < PIER_RAYS; i++) {
+            const float4 ray = pier_ray(sc[2 * ai], sc[2 * ai + 1], ps.sq.x + ps.dx * (float)i, ps.r0, ps.r1);
+            if (polygon_has_point(shape, ray.x, ray.y) || polygon_has_point(shape, ray.z, ray.w)) return false;
             // clip (:124-151): intersections sorted by distance from the ray start
             std::vector<std::pair<float, float2>> hits;
             for (uint32_t k = 0; k < shape.n(); k++) {
                 const float4 e = shape.edge(k);
-                if (!edge_edge(e, make_float4(a.x, a.y, b.x, b.y))) continue;
-                const float x1 = e.x, y1 = e.y, x2 = e.z, y2 = e.w, x3 = a.x, y3 = a.y, x4 = b.x, y4 = b.y;
-                const float t = ((x2 - x4) * (y4 - y3) - (y2 - y4) * (x4 - x3)) /
-                                ((x2 - x1) * (y4 - y3) - (y2 - y1) * (x4 - x3));
-                const float2 loc = make_float2(x2 + t * (x1 - x2), y2 + t * (y1 - y2));
-                hits.push_back({sqrtf(sq_dist(a, loc)), loc});
+                if (!edge_edge(e, ray)) continue;
+                float d;
+                const float2 loc = pier_hit(e, ray, d);
+                hits.push_back({d, loc});
             }
             std::stable_sort(hits.begin(), hits.end(),
                              [](const std::pair<float, float2>& u, const std::pair<float, float2>& w) { return u.first < w.first; });
             for (size_t k = 0; k + 1 < hits.size(); k += 2) {
                 const float2 st = hits[k].second, en = hits[k + 1].second;
                 if (st.x == en.x && st.y == en.y) continue;
-                const float ddx = en.x - st.x, ddy = en.y - st.y;   // Edge::scale (edge.rs:39-46)
-                clipped.push_back(make_float4(st.x - ddx * (TRIM - 1.0f) / 2.0f, st.y - ddy * (TRIM - 1.0f) / 2.0f,
-                                              en.x + ddx * (TRIM - 1.0f) / 2.0f, en.y + ddy * (TRIM - 1.0f) / 2.0f));
+                clipped.push_back(pier_trim(st, en));
             }
         }
-    }
     std::vector<float2> grid;
-    for (size_t i = 0; i < GRID; i++)
-        for (size_t j = 0; j < GRID; j++) {
-            const float2 p = make_float2(lin(sq.x, sq.z, GRID, i), lin(sq.y, sq.w, GRID, j));
-            if (!polygon_has_point(shape, p.x, p.y)) continue;
-            bool free = true;
-            for (const float4& c : poles)
-                if (sq_dist(p, make_float2(c.x, c.y)) <= c.z * c.z) { free = false; break; }
-            if (free) grid.push_back(p);
-        }
-    const float infl = INFL * (sq.z - sq.x);
-    const float bw = bb.z - bb.x, bh = bb.w - bb.y;
-    const float forfeit = sqrtf((bw * bw) * (bh * bh));
-    for (uint32_t it = 0; it < n; it++) {
-        std::vector<float> d_ray(grid.size()), d_pole(grid.size());
-        for (size_t g = 0; g < grid.size(); g++) {
-            float a = forfeit, b = forfeit;
-            for (const float4& r : out) a = mn(a, edge_point_dist(r, grid[g].x, grid[g].y));
-            for (const float4& c : poles) {      // Circle::distance_to (circle.rs:136-147)
-                const float ddx = grid[g].x - c.x, ddy = grid[g].y - c.y;
-                const float sqd = ddx * ddx + ddy * ddy;
-                b = mn(b, sqd < c.z * c.z ? 0.0f : sqrtf(sqd) - c.z);
+    std::vector<float> d_pole;
+    for (uint32_t i = 0; i < PIER_GRID; i++)
+        for (uint32_t j = 0; j < PIER_GRID; j++) {
+            float dp;
+            float2 p;
+            if (pier_grid_point(shape.v.data(), shape.n(), shape.bbox, poles.data(), (uint32_t)poles.size(), ps, i, j, p, dp)) {
+                grid.push_back(p);
+                d_pole.push_back(dp);
             }
-            d_ray[g] = a;
-            d_pole[g] = b;
         }
+    std::vector<float> d_ray(grid.size(), ps.forfeit);
+    for (uint32_t it = 0; it < n; it++) {
         if (clipped.empty()) return false;
         size_t best_i = 0;
         float best_loss = 0.0f;
         for (size_t r = 0; r < clipped.size(); r++) {
             float loss = -0.0f;
-            for (size_t g = 0; g < grid.size(); g++) {
-                const float dn = mn(d_ray[g], edge_point_dist(clipped[r], grid[g].x, grid[g].y));
-                const float v = dn < infl ? mn(d_pole[g], dn) : d_pole[g];
-                loss += v * v;
-            }
+            for (size_t g = 0; g < grid.size(); g++)
+                loss += pier_point_loss(d_ray[g], d_pole[g], edge_point_dist(clipped[r], grid[g].x, grid[g].y), ps.infl);
             if (r == 0 || loss < best_loss) { best_loss = loss; best_i = r; }
         }
         out.push_back(clipped[best_i]);
+        // min_distances_to_rays (:215-225) folds min over the chosen piers; min is exact, so
+        // folding in the new pier gives the same values
+        for (size_t g = 0; g < grid.size(); g++)
+            d_ray[g] = mn(d_ray[g], edge_point_dist(clipped[best_i], grid[g].x, grid[g].y));
     }
     return true;
 }
@@ -283,7 +272,8 @@ bool polygon_generate_surrogate(Polygon& p, const jg_surrogate_config& cfg, std:
     return surrogate_finish(p, cfg, std::move(poles), err);
 }
 
-bool surrogate_finish(Polygon& p, const jg_surrogate_config& cfg, std::vector<float4> poles, std::string& err) {
+bool surrogate_finish(Polygon& p, const jg_surrogate_config& cfg, std::vector<float4> poles, std::string& err,
+                      const std::vector<float4>* piers) {
     Surrogate s;
     std::vector<float2> h = hull_points(p.v);
     for (const float2& q : h)
@@ -293,7 +283,8 @@ bool surrogate_finish(Polygon& p, const jg_surrogate_config& cfg, std::vector<fl
     s.poles = std::move(poles);
     const size_t n_ff = std::min<size_t>(cfg.n_ff_poles, s.poles.size());
     std::vector<float4> rel(s.poles.begin(), s.poles.begin() + (long)n_ff);
-    if (!surrogate_piers(p, cfg.n_ff_piers, rel, s.piers)) { err = "pier generation failed"; return false; }
+    if (piers) s.piers = *piers;
+    else if (!surrogate_piers(p, cfg.n_ff_piers, rel, s.piers)) { err = "pier generation failed"; return false; }
     s.cfg = cfg;
     s.present = true;
     p.sur = std::move(s);

@@ -41,8 +41,13 @@ bool polygon_generate_surrogate(Polygon& p, const jg_surrogate_config& cfg, std:
 // SPolygon::new without the pole of inaccessibility (poi left unset), and the surrogate from
 // its poles (hull, piers)
 bool polygon_prepare(const float* xy, uint32_t n, Polygon& out, std::string& err);
-bool surrogate_finish(Polygon& p, const jg_surrogate_config& cfg, std::vector<float4> poles, std::string& err);
+// piers, when given, replace the host pier search (they come from the device's)
+bool surrogate_finish(Polygon& p, const jg_surrogate_config& cfg, std::vector<float4> poles, std::string& err,
+                      const std::vector<float4>* piers = nullptr);
 Polygon polygon_transform(const Polygon& p, const Mat& m);
+// generate_piers' per-polygon constants and the (sin, cos) of its 90 ray angles (host libm)
+void pier_setup(const Polygon& shape, PierSetup& out);
+void pier_angles(float* sin_cos);
 bool polygon_has_point(const Polygon& p, float x, float y);
 Mat mat_from_dt(float rotation, float tx, float ty);
 

@@ -60,4 +60,9 @@ jg_status item_upload(jg_spolygon* it);
 jg_status filter_view(jg_cde* c, const jg_hazard_filter* f, FilterView& out);
 jg_status plan_prepare(jg_plan* p);
 jg_status plan_enqueue(jg_plan* p, const float* d_m6, uint8_t* d_status, hipStream_t stream);
+// Piers of many polygons on the GPU (generate_piers, fail_fast/piers.rs:25-98), bit for bit
+// with the host routine; poles[i] are polygon i's relevant poles.  host[i] is set where the
+// device could not finish the search the way the reference would; those take the host routine.
+jg_status piers_batch(const std::vector<const Polygon*>& polys, const std::vector<std::vector<float4>>& poles,
+                      uint32_t n_piers, std::vector<std::vector<float4>>& out, std::vector<uint8_t>& host);
 }  // namespace jg

@@ -294,6 +294,26 @@ extern "C" jg_status jg_spolygon_new_batch(const float* xy, const uint32_t* offs
         if (status[i] == PJ_FAIL) return fail(JG_ERR_POLYGON, "polygon " + std::to_string(i) + ": no pole found");
         if (status[i] == PJ_HOST) n_host++;
     }
+    // piers of the polygons whose poles came from the device, on the device as well
+    std::vector<std::vector<float4>> piers;
+    std::vector<uint8_t> pier_host;
+    std::vector<uint32_t> pier_of(n_polys, 0xffffffffu);
+    if (cfg && cfg->n_ff_piers > 0) {
+        std::vector<const Polygon*> pp;
+        std::vector<std::vector<float4>> rel;
+        for (uint32_t i = 0; i < n_polys; i++) {
+            if (status[i] != PJ_OK) continue;
+            const float4* P = poles.data() + (size_t)i * pole_cap;
+            const uint32_t n_ff = std::min(cfg->n_ff_poles, np[i]);
+            polys[i].poi = P[0];
+            pier_of[i] = (uint32_t)pp.size();
+            pp.push_back(&polys[i]);
+            rel.emplace_back(P, P + n_ff);
+        }
+        st = piers_batch(pp, rel, cfg->n_ff_piers, piers, pier_host);
+        if (st != JG_OK) return st;
+        for (uint8_t h : pier_host) n_host += h;
+    }
     parallel_for(n_polys, [&](size_t i) {
         Polygon& p = polys[i];
         std::string& err = errs[i];
@@ -309,7 +329,9 @@ extern "C" jg_status jg_spolygon_new_batch(const float* xy, const uint32_t* offs
         }
         const float4* P = poles.data() + (size_t)i * pole_cap;
         p.poi = P[0];
-        if (cfg && !surrogate_finish(p, *cfg, std::vector<float4>(P, P + np[i]), err)) ok[i] = 0;
+        const std::vector<float4>* dev_piers =
+            pier_of[i] != 0xffffffffu && !pier_host[pier_of[i]] ? &piers[pier_of[i]] : nullptr;
+        if (cfg && !surrogate_finish(p, *cfg, std::vector<float4>(P, P + np[i]), err, dev_piers)) ok[i] = 0;
     });
     for (uint32_t i = 0; i < n_polys; i++)
         if (!ok[i]) return fail(JG_ERR_SURROGATE, "polygon " + std::to_string(i) + ": " + errs[i]);

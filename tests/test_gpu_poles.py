@@ -59,3 +59,36 @@ def test_batch_errors(cuda):
         J.SPolygon.new_batch(ps)
     got, n = J.SPolygon.new_batch([])
     assert got == [] and n == 0
+
+
+def _comb(teeth=20):
+    """A comb (teeth [2k, 2k+1] x [5, 20] on a base [0, 2 teeth - 1] x [0, 5]) whose horizontal
+    rays cross its boundary 2 * teeth times."""
+    pts = [(0.0, 0.0), (2.0 * teeth - 1.0, 0.0)]
+    for k in range(teeth - 1, -1, -1):
+        if k < teeth - 1:
+            pts.append((2.0 * k + 1.0, 5.0))
+        pts += [(2.0 * k + 1.0, 20.0), (2.0 * k, 20.0)]
+        if k > 0:
+            pts.append((2.0 * k, 5.0))
+    return np.array(pts, np.float32)
+
+
+@pytest.mark.parametrize("n_piers,n_ff", [(1, 1), (3, 2)])
+def test_batch_piers_match_host(cuda, n_piers, n_ff):
+    """generate_piers on the GPU (ray clipping, the point lattice, sequential loss sums, argmin
+    with ties to the first clip, distance updates) equals the host routine bit for bit.  Two
+    stars have rays crossing their boundary 40 and 48 times; the comb's pole search outgrows
+    the device frontier, so that polygon is finished on the host."""
+    import concurrent.futures as cf
+    ps = [S.polygon(40_000 + i, 8, 48, 5.0, 30.0, i % 3) for i in range(20)] + [_comb()]
+    ps += [S.polygon(50_000 + i, 8, 256, 5.0, 30.0, i % 3) for i in (324, 429)]
+    cfg = J.SPSurrogateConfig(n_ff_poles=n_ff, n_ff_piers=n_piers)
+    got, n_host = J.SPolygon.new_batch(ps, cfg)
+    assert n_host == 1
+    with cf.ThreadPoolExecutor(8) as ex:
+        ref = list(ex.map(lambda p: J.SPolygon.new(p).generate_surrogate(cfg), ps))
+    for g, r in zip(got, ref):
+        assert g.piers.shape == (n_piers, 4)
+        _same(g, r)
+    assert_same_item(got[0], ps[0], n_ff_poles=n_ff, n_ff_piers=n_piers)
