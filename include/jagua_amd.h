@@ -149,10 +149,20 @@ jg_status jg_spolygon_get_info(const jg_spolygon* poly, jg_spolygon_info* out);
 jg_status jg_spolygon_get_vertices(const jg_spolygon* poly, float* xy_out);       /* [n][2] */
 jg_status jg_spolygon_get_poles(const jg_spolygon* poly, float* out);             /* [n_poles][3] */
 jg_status jg_spolygon_get_piers(const jg_spolygon* poly, float* out);             /* [n_piers][4] */
+/* SPolygon::centroid (simple_polygon.rs:173-192). */
+jg_status jg_spolygon_centroid(const jg_spolygon* poly, float out[2]);
 
 /* DTransformation::compose (d_transformation.rs:45-47 -> transformation.rs:91-101,131-139):
  * dt[i] = (rotation, tx, ty) -> mat6[i]; sin/cos come from the host libm (as in Rust std). */
 jg_status jg_transforms_from_dt(const float* dt, uint64_t n, float* mat6_out);
+
+/* The transformation chain of the instance import and export: out = decompose(rot_transl(b) .
+ * X . I) with X = rot_transl(a), or its inverse when invert_a (transformation.rs:60-88,
+ * 132-210, all products in f32 in written order; decompose takes atan2f).  invert_a = 1 is
+ * ext_to_int_transformation(ext = b, pre = a) (io/import.rs:222-233); invert_a = 0 is
+ * int_to_ext_transformation(int = b, pre = a) (io/export.rs:34-45).  Transformations are
+ * (rotation in radians, tx, ty). */
+jg_status jg_dt_chain(const float a[3], uint32_t invert_a, const float b[3], float out[3]);
 
 /* ---------------------------------------------------------------- CDEngine */
 

@@ -565,6 +565,20 @@ extern "C" jg_status jg_transforms_from_dt(const float* dt, uint64_t n, float* o
     return JG_OK;
 }
 
+extern "C" jg_status jg_dt_chain(const float a[3], uint32_t invert_a, const float b[3], float out[3]) {
+    if (!a || !b || !out) return fail(JG_ERR_INVALID_ARG, "null argument");
+    if (!dt_chain(a, invert_a != 0, b, out)) return fail(JG_ERR_INVALID_ARG, "transformation is NaN");
+    return JG_OK;
+}
+
+extern "C" jg_status jg_spolygon_centroid(const jg_spolygon* p, float out[2]) {
+    if (!p || !out) return fail(JG_ERR_INVALID_ARG, "null argument");
+    const float2 c = polygon_centroid(p->poly);
+    out[0] = c.x;
+    out[1] = c.y;
+    return JG_OK;
+}
+
 // ------------------------------------------------------------------------------ ABI: engine
 static jg_hazkey cde_register(jg_cde* c, const jg_hazard_entity& ent, const Polygon& shape, bool dynamic) {
     HazardRec r;

@@ -22,6 +22,50 @@ Mat mat_from_dt(float rotation, float tx, float ty) {
     return Mat{c, -s, tx, s, c, ty};
 }
 
+// Transformation algebra (transformation.rs:104-210): 3x3 matrices in f32, every product and
+// sum in the written order (products with the identity included: they decide zero signs).
+using M3 = float[3][3];
+static void rot_transl_m(float angle, float tx, float ty, M3 m) {     // :132-141
+    const float s = sinf(angle), c = cosf(angle);
+    const float r[3][3] = {{c, -s, tx}, {s, c, ty}, {0.0f, 0.0f, 1.0f}};
+    std::memcpy(m, r, sizeof r);
+}
+static void dot_prod(const M3 l, const M3 r, M3 o) {                 // :158-179
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) o[i][j] = l[i][0] * r[0][j] + l[i][1] * r[1][j] + l[i][2] * r[2][j];
+}
+static void inverse3(const M3 m, M3 o) {                              // :181-210
+    const float det = m[0][0] * m[1][1] * m[2][2] + m[0][1] * m[1][2] * m[2][0] + m[0][2] * m[1][0] * m[2][1] -
+                      m[0][2] * m[1][1] * m[2][0] - m[0][1] * m[1][0] * m[2][2] - m[0][0] * m[1][2] * m[2][1];
+    o[0][0] = (m[1][1] * m[2][2] - m[1][2] * m[2][1]) / det;
+    o[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) / det;
+    o[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) / det;
+    o[1][0] = (m[1][2] * m[2][0] - m[1][0] * m[2][2]) / det;
+    o[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) / det;
+    o[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) / det;
+    o[2][0] = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) / det;
+    o[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) / det;
+    o[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) / det;
+}
+
+bool dt_chain(const float a[3], bool invert_a, const float b[3], float out[3]) {
+    M3 A, X, T1, B, T2;
+    const M3 I = {{1.0f, 0.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, {0.0f, 0.0f, 1.0f}};
+    rot_transl_m(a[0], a[1], a[2], A);
+    if (invert_a) inverse3(A, X);
+    else std::memcpy(X, A, sizeof A);
+    dot_prod(X, I, T1);                      // Transformation::empty().transform(X) (:60-63)
+    rot_transl_m(b[0], b[1], b[2], B);
+    dot_prod(B, T1, T2);                     // .transform_from_decomposed(b) (:48-51, :65-67)
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            if (T2[i][j] != T2[i][j]) return false;   // NotNan arithmetic would panic
+    out[0] = atan2f(T2[1][0], T2[0][0]);     // decompose (:83-88)
+    out[1] = T2[0][2];
+    out[2] = T2[1][2];
+    return true;
+}
+
 static float signed_area(const std::vector<float2>& v) {           // simple_polygon.rs:137-150
     float sigma = 0.0f;
     const size_t n = v.size();
@@ -109,9 +153,9 @@ bool polygon_prepare(const float* xy, uint32_t n, Polygon& out, std::string& err
     if (n < 3) { err = "simple polygon must have at least 3 points"; return false; }
     std::vector<float2> v(n);
     for (uint32_t i = 0; i < n; i++) v[i] = make_float2(xy[2 * i], xy[2 * i + 1]);
-    {
-        std::vector<std::pair<float, float>> s(n);
-        for (uint32_t i = 0; i < n; i++) s[i] = {v[i].x, v[i].y};
+    {   // itertools unique() hashes Point by its bit patterns (point.rs:57-63): 0.0 and -0.0 differ
+        std::vector<uint64_t> s(n);
+        for (uint32_t i = 0; i < n; i++) s[i] = (uint64_t)f2u(v[i].x) << 32 | f2u(v[i].y);
         std::sort(s.begin(), s.end());
         for (uint32_t i = 1; i < n; i++)
             if (s[i] == s[i - 1]) { err = "simple polygon should not contain duplicate points"; return false; }
@@ -171,7 +215,7 @@ static bool surrogate_poles(const Polygon& shape, const jg_surrogate_config& cfg
 }
 
 // SPolygon::centroid (simple_polygon.rs:173-192)
-static float2 polygon_centroid(const Polygon& shape) {
+float2 polygon_centroid(const Polygon& shape) {
     float gx = 0.0f, gy = 0.0f;
     for (uint32_t i = 0; i < shape.n(); i++) {
         const float2 a = shape.v[i], b = shape.v[i + 1 == shape.n() ? 0 : i + 1];

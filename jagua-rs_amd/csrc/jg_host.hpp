@@ -45,6 +45,11 @@ bool polygon_prepare(const float* xy, uint32_t n, Polygon& out, std::string& err
 bool surrogate_finish(Polygon& p, const jg_surrogate_config& cfg, std::vector<float4> poles, std::string& err,
                       const std::vector<float4>* piers = nullptr);
 Polygon polygon_transform(const Polygon& p, const Mat& m);
+// SPolygon::centroid (simple_polygon.rs:173-192)
+float2 polygon_centroid(const Polygon& shape);
+// decompose(rot_transl(b) . X . I) with X = rot_transl(a), inverted when invert_a
+// (transformation.rs); false when the reference's NotNan arithmetic would panic.
+bool dt_chain(const float a[3], bool invert_a, const float b[3], float out[3]);
 // generate_piers' per-polygon constants and the (sin, cos) of its 90 ray angles (host libm)
 void pier_setup(const Polygon& shape, PierSetup& out);
 void pier_angles(float* sin_cos);

@@ -89,6 +89,8 @@ _SIGS = {
     "jg_spolygon_get_poles": (C.c_int32, [c_vp, c_f32p]),
     "jg_spolygon_get_piers": (C.c_int32, [c_vp, c_f32p]),
     "jg_transforms_from_dt": (C.c_int32, [c_f32p, C.c_uint64, c_f32p]),
+    "jg_dt_chain": (C.c_int32, [c_f32p, C.c_uint32, c_f32p, c_f32p]),
+    "jg_spolygon_centroid": (C.c_int32, [c_vp, c_f32p]),
     "jg_cde_new": (C.c_int32, [c_f32p, C.POINTER(CDEConfigC), C.POINTER(HazardEntityC), C.POINTER(c_vp),
                                c_u8p, C.c_uint32, C.POINTER(c_vp)]),
     "jg_cde_free": (None, [c_vp]),

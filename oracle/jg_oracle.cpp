@@ -177,7 +177,7 @@ SPolygon spolygon_new(std::vector<Point> pts) {
     if (pts.size() < 3) throw std::runtime_error("simple polygon must have at least 3 points");
     for (size_t i = 0; i < pts.size(); i++)
         for (size_t j = i + 1; j < pts.size(); j++)
-            if (pts[i].x == pts[j].x && pts[i].y == pts[j].y)
+            if (bits(pts[i].x) == bits(pts[j].x) && bits(pts[i].y) == bits(pts[j].y))   // unique() hashes bits
                 throw std::runtime_error("simple polygon should not contain duplicate points");
     float area = spolygon_calculate_area(pts);
     if (area == 0.0f) throw std::runtime_error("simple polygon has no area");
@@ -998,6 +998,77 @@ bool qt_nodes_match(const QTNode* a, const QTNode* b) {
     for (int i = 0; i < 4; i++)
         if (!qt_nodes_match(&(*ca)[i], &(*cb)[i])) return false;
     return true;
+}
+
+// ---------------------------------------------------------------------------------------------
+// instance import (io/import.rs) and the transformation algebra it uses (transformation.rs)
+// ---------------------------------------------------------------------------------------------
+std::vector<Point> import_simple_polygon_points(std::vector<Point> pts) {
+    if (pts.size() > 1 && pts.front().x == pts.back().x && pts.front().y == pts.back().y) pts.pop_back();
+    // eliminate_degenerate_vertices (:235-260): mark i when point i ~ point i+1 (cyclic), remove
+    // the marks from the highest index down
+    const size_t n = pts.size();
+    std::vector<size_t> marks;
+    for (size_t i = 0; i < n; i++) {
+        const size_t j = (i + 1) % n;
+        if (fpa_eq(pts[i].x, pts[j].x) && fpa_eq(pts[i].y, pts[j].y)) marks.push_back(i);
+    }
+    std::sort(marks.rbegin(), marks.rend());
+    for (size_t i : marks)
+        if (i < pts.size()) pts.erase(pts.begin() + (long)i);
+    for (size_t i = 0; i < pts.size(); i++)
+        for (size_t j = i + 1; j < pts.size(); j++)
+            if (bits(pts[i].x) == bits(pts[j].x) && bits(pts[i].y) == bits(pts[j].y))
+                throw std::runtime_error("Simple polygon has non-consecutive duplicate vertices");
+    return pts;
+}
+
+Mat3 mat3_rot_transl(float angle, float tx, float ty) {
+    const float sin = std::sin(angle), cos = std::cos(angle);
+    return Mat3{{{cos, -sin, tx}, {sin, cos, ty}, {0.0f, 0.0f, 1.0f}}};
+}
+
+Mat3 mat3_dot(const Mat3& l, const Mat3& r) {
+    Mat3 o;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) o.m[i][j] = l.m[i][0] * r.m[0][j] + l.m[i][1] * r.m[1][j] + l.m[i][2] * r.m[2][j];
+    return o;
+}
+
+Mat3 mat3_inverse(const Mat3& a) {
+    const auto& m = a.m;
+    const float det = m[0][0] * m[1][1] * m[2][2] + m[0][1] * m[1][2] * m[2][0] + m[0][2] * m[1][0] * m[2][1] -
+                      m[0][2] * m[1][1] * m[2][0] - m[0][1] * m[1][0] * m[2][2] - m[0][0] * m[1][2] * m[2][1];
+    return Mat3{{{(m[1][1] * m[2][2] - m[1][2] * m[2][1]) / det, (m[0][2] * m[2][1] - m[0][1] * m[2][2]) / det,
+                  (m[0][1] * m[1][2] - m[0][2] * m[1][1]) / det},
+                 {(m[1][2] * m[2][0] - m[1][0] * m[2][2]) / det, (m[0][0] * m[2][2] - m[0][2] * m[2][0]) / det,
+                  (m[0][2] * m[1][0] - m[0][0] * m[1][2]) / det},
+                 {(m[1][0] * m[2][1] - m[1][1] * m[2][0]) / det, (m[0][1] * m[2][0] - m[0][0] * m[2][1]) / det,
+                  (m[0][0] * m[1][1] - m[0][1] * m[1][0]) / det}}};
+}
+
+static const Mat3 MAT3_EMPTY{{{1.0f, 0.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, {0.0f, 0.0f, 1.0f}}};
+
+static bool decompose(const Mat3& t, float out[3]) {                 // transformation.rs:83-88
+    for (auto& row : t.m)
+        for (float v : row)
+            if (v != v) return false;
+    out[0] = std::atan2(t.m[1][0], t.m[0][0]);
+    out[1] = t.m[0][2];
+    out[2] = t.m[1][2];
+    return true;
+}
+
+bool ext_to_int_transformation(const float ext[3], const float pre[3], float out[3]) {
+    // Transformation::empty().transform(&pre.compose().inverse()).transform_from_decomposed(ext)
+    const Mat3 t1 = mat3_dot(mat3_inverse(mat3_rot_transl(pre[0], pre[1], pre[2])), MAT3_EMPTY);
+    return decompose(mat3_dot(mat3_rot_transl(ext[0], ext[1], ext[2]), t1), out);
+}
+
+bool int_to_ext_transformation(const float in[3], const float pre[3], float out[3]) {
+    // Transformation::empty().transform_from_decomposed(pre).transform_from_decomposed(int)
+    const Mat3 t1 = mat3_dot(mat3_rot_transl(pre[0], pre[1], pre[2]), MAT3_EMPTY);
+    return decompose(mat3_dot(mat3_rot_transl(in[0], in[1], in[2]), t1), out);
 }
 
 }  // namespace oracle

@@ -38,6 +38,12 @@ enum class GeoPosition { Exterior, Interior };                              // g
 inline float fmin32(float a, float b) { return std::fmin(a, b); }
 inline float fmax32(float a, float b) { return std::fmax(a, b); }
 
+// f32::to_bits
+inline uint32_t bits(float a) {
+    uint32_t u;
+    std::memcpy(&u, &a, 4);
+    return u;
+}
 // util/fpa.rs:18-31 — float_cmp::approx_eq!(f32, a, b) with the default F32Margin
 // (epsilon = f32::EPSILON, ulps = 4): exact ==, then |a-b| <= EPSILON, then
 // saturating_abs(bits(a) wrapping_sub bits(b)) <= 4.
@@ -290,6 +296,20 @@ float spolygon_calculate_diameter(std::vector<Point> points);           // :107-
 Point spolygon_centroid(const SPolygon& s);                             // :173-192
 bool spolygon_collides_point(const SPolygon& s, const Point& p);        // :255-292
 SPolygon spolygon_transform_clone(const SPolygon& s, const Transformation& t);  // :195-224
+
+// ---- instance import (io/import.rs)
+// import_simple_polygon (:198-212) up to SPolygon::new: closing vertex stripped, consecutive
+// near-duplicates eliminated (:235-260), non-consecutive duplicates rejected (throws).
+std::vector<Point> import_simple_polygon_points(std::vector<Point> pts);
+// Transformation as the full 3x3 matrix of transformation.rs
+struct Mat3 { float m[3][3]; };
+Mat3 mat3_rot_transl(float angle, float tx, float ty);   // rot_transl_m (:132-141)
+Mat3 mat3_dot(const Mat3& l, const Mat3& r);             // dot_prod (:158-179)
+Mat3 mat3_inverse(const Mat3& m);                        // inverse (:181-210)
+// ext_to_int_transformation (io/import.rs:222-233) and int_to_ext_transformation
+// (io/export.rs:34-45); dt = (rotation, tx, ty); false where NotNan arithmetic would panic
+bool ext_to_int_transformation(const float ext[3], const float pre[3], float out[3]);
+bool int_to_ext_transformation(const float in[3], const float pre[3], float out[3]);
 std::vector<Point> convex_hull_from_points(std::vector<Point> points);  // convex_hull.rs:31-64
 Circle compute_pole(const SPolygon& shape, const std::vector<Circle>& poles);  // pole.rs:52-77
 std::vector<Circle> generate_surrogate_poles(const SPolygon& shape, const SPSurrogateConfig& cfg);  // pole.rs:12-47

@@ -19,7 +19,8 @@ u64p = C.POINTER(C.c_uint64)
 
 class HazardDesc(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("n_verts", C.c_uint32), ("a", C.c_uint64), ("b", C.c_uint64),
-                ("dt", C.c_float * 3), ("dynamic", C.c_uint32), ("has_dt", C.c_uint32), ("_pad", C.c_uint32)]
+                ("dt", C.c_float * 3), ("dynamic", C.c_uint32), ("has_dt", C.c_uint32), ("has_pre", C.c_uint32),
+                ("pre", C.c_float * 3), ("_pad", C.c_uint32)]
 
 
 class Filter(C.Structure):
@@ -60,6 +61,12 @@ def lib():
         L.or_cde_dump.argtypes = [C.c_void_p, u32p, C.c_uint64, u32p, C.c_uint64, u64p]
         L.or_item_create.restype = C.c_void_p
         L.or_item_create.argtypes = [f32p, C.c_uint32, u32p, f32p, C.c_uint32, C.c_uint32]
+        L.or_item_create_pre.restype = C.c_void_p
+        L.or_item_create_pre.argtypes = [f32p, C.c_uint32, f32p, u32p, f32p, C.c_uint32, C.c_uint32]
+        L.or_import_polygon.argtypes = [f32p, C.c_uint32, f32p, f32p]
+        L.or_item_info.argtypes = [C.c_void_p, f32p, C.c_uint32, f32p, f32p, C.c_uint32, u32p]
+        L.or_ext_to_int.argtypes = [f32p, f32p, f32p]
+        L.or_int_to_ext.argtypes = [f32p, f32p, f32p]
         L.or_item_from_parts.restype = C.c_void_p
         L.or_item_from_parts.argtypes = [f32p, C.c_uint32, f32p, f32p, C.c_uint32, f32p, C.c_uint32, C.c_uint32]
         L.or_item_destroy.argtypes = [C.c_void_p]
@@ -135,19 +142,38 @@ class OracleCDE:
         for i, (loc, dt) in enumerate(scene.items):
             self.keys.append(self.register_item(i, loc, dt, pk=i))
 
+    @classmethod
+    def from_hazards(cls, bbox, depth, cd_threshold, hazards):
+        """An engine from static hazards [(kind, a, b, vertices)], in order."""
+        self = cls.__new__(cls)
+        descs = [cls._desc(k, xy, a=a, b=b) for k, a, b, xy in hazards]
+        arr = (HazardDesc * len(descs))(*descs)
+        v = np.concatenate([_f(xy).ravel() for _, _, _, xy in hazards])
+        self.h = lib().or_cde_create(fp(_f(bbox)), depth, cd_threshold, len(descs), arr, fp(v))
+        if not self.h:
+            raise RuntimeError("oracle CDE construction failed")
+        self.keys = []
+        return self
+
     @staticmethod
-    def _desc(kind, xy, a=0, b=0, dt=None, dynamic=False):
+    def _desc(kind, xy, a=0, b=0, dt=None, dynamic=False, pre=None):
         d = HazardDesc()
         d.kind, d.n_verts, d.a, d.b = kind, len(xy), a, b
         if dt is not None:
             for i in range(3):
                 d.dt[i] = float(dt[i])
             d.has_dt = 1
+        if pre is not None:
+            for i in range(3):
+                d.pre[i] = float(pre[i])
+            d.has_pre = 1
         d.dynamic = 1 if dynamic else 0
         return d
 
-    def register_item(self, item_id, local, dt, pk):
-        d = self._desc(KIND_PLACED, local, a=item_id, b=pk, dt=dt, dynamic=True)
+    def register_item(self, item_id, local, dt, pk, pre=None):
+        """A placed item: SPolygon::new(local), transformed by pre (an imported item's
+        centering) and then by dt."""
+        d = self._desc(KIND_PLACED, local, a=item_id, b=pk, dt=dt, dynamic=True, pre=pre)
         key = (C.c_uint32 * 2)()
         loc = _f(local)
         if lib().or_cde_register(self.h, C.byref(d), fp(loc), key) != 0:
@@ -188,13 +214,28 @@ class OracleCDE:
 
 
 class OracleItem:
-    def __init__(self, xy, limits=DEFAULT_LIMITS, n_ff_poles=2, n_ff_piers=0):
+    def __init__(self, xy, limits=DEFAULT_LIMITS, n_ff_poles=2, n_ff_piers=0, pre=None):
+        """SPolygon::new(xy) (transformed by pre first when given: an imported item) with
+        its surrogate."""
         xy = _f(xy).reshape(-1, 2)
         ln = (C.c_uint32 * 3)(*[n for n, _ in limits])
         lc = (C.c_float * 3)(*[c for _, c in limits])
-        self.h = lib().or_item_create(fp(xy), len(xy), ln, lc, n_ff_poles, n_ff_piers)
+        if pre is None:
+            self.h = lib().or_item_create(fp(xy), len(xy), ln, lc, n_ff_poles, n_ff_piers)
+        else:
+            self.h = lib().or_item_create_pre(fp(xy), len(xy), fp(_f(pre)), ln, lc, n_ff_poles, n_ff_piers)
+        self._n = len(xy)
         if not self.h:
             raise RuntimeError("oracle item construction failed")
+
+    def info(self):
+        """(vertices, meta [area, diameter, bbox (4), poi (3)], poles [n][3]) of the item's shape."""
+        xy = np.zeros((self._n, 2), dtype=np.float32)
+        meta = np.zeros(9, dtype=np.float32)
+        poles = np.zeros((1000, 3), dtype=np.float32)
+        npl = C.c_uint32()
+        lib().or_item_info(self.h, fp(xy), self._n, fp(meta), fp(poles), 1000, C.byref(npl))
+        return xy, meta, poles[: npl.value]
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -247,3 +288,27 @@ def overlap64(pose, poles, off, pairs, cedges, grad=True):
                        pairs.ctypes.data_as(u32p), len(pairs), fp(cedges), len(cedges), C.byref(q),
                        g.ctypes.data_as(C.POINTER(C.c_double)) if grad else None)
     return q.value, g
+
+
+def import_polygon(xy):
+    """import_simple_polygon (io/import.rs:198-212) + SPolygon::new: (vertices, centroid), or
+    None when the reference bails."""
+    xy = _f(xy).reshape(-1, 2)
+    out = np.zeros_like(xy)
+    c = np.zeros(2, dtype=np.float32)
+    n = lib().or_import_polygon(fp(xy), len(xy), fp(out), fp(c))
+    return None if n < 0 else (out[:n], c)
+
+
+def ext_to_int(ext, pre):
+    out = np.zeros(3, dtype=np.float32)
+    if lib().or_ext_to_int(fp(_f(ext)), fp(_f(pre)), fp(out)) != 0:
+        raise RuntimeError("NaN transformation")
+    return out
+
+
+def int_to_ext(int_dt, pre):
+    out = np.zeros(3, dtype=np.float32)
+    if lib().or_int_to_ext(fp(_f(int_dt)), fp(_f(pre)), fp(out)) != 0:
+        raise RuntimeError("NaN transformation")
+    return out

@@ -68,6 +68,8 @@ struct or_hazard_desc {
     float dt[3];
     uint32_t dynamic;
     uint32_t has_dt;     // placed items: shape = SPolygon::new(local) transformed by dt
+    uint32_t has_pre;    // imported items: transformed by pre (the centering) before dt
+    float pre[3];
     uint32_t _pad;
 };
 
@@ -154,6 +156,7 @@ static Hazard make_hazard(const or_hazard_desc& d, const float* verts) {
     std::memcpy(h.entity.dt, d.dt, sizeof d.dt);
     h.dynamic = d.dynamic != 0;
     SPolygon s = spolygon_new(to_points(verts, d.n_verts));
+    if (d.has_pre) s = spolygon_transform_clone(s, transformation_from_dt(d.pre[0], d.pre[1], d.pre[2]));
     if (d.has_dt) s = spolygon_transform_clone(s, transformation_from_dt(d.dt[0], d.dt[1], d.dt[2]));
     h.shape = std::move(s);
     return h;
@@ -291,6 +294,38 @@ void* or_item_create(const float* xy, uint32_t n, const uint32_t* lim_n, const f
         return nullptr;
     }
 }
+// An imported item's internal shape (Item::new, entities/item.rs:22-42): SPolygon::new of the
+// original, transformed by pre (the centering), then the surrogate.
+void* or_item_create_pre(const float* xy, uint32_t n, const float* pre, const uint32_t* lim_n, const float* lim_cov,
+                         uint32_t n_ff_poles, uint32_t n_ff_piers) {
+    try {
+        auto* it = new OrItem;
+        it->shape = spolygon_transform_clone(spolygon_new(to_points(xy, n)), transformation_from_dt(pre[0], pre[1], pre[2]));
+        spolygon_generate_surrogate(it->shape, make_cfg(lim_n, lim_cov, n_ff_poles, n_ff_piers));
+        return it;
+    } catch (...) {
+        return nullptr;
+    }
+}
+
+// import_simple_polygon's vertices (io/import.rs:198-212) passed through SPolygon::new; the
+// centroid of the polygon (simple_polygon.rs:173-192).  Returns the vertex count, -1 on error.
+int or_import_polygon(const float* xy, uint32_t n, float* out_xy, float* centroid2) {
+    try {
+        SPolygon s = spolygon_new(import_simple_polygon_points(to_points(xy, n)));
+        for (size_t i = 0; i < s.vertices.size(); i++) { out_xy[2 * i] = s.vertices[i].x; out_xy[2 * i + 1] = s.vertices[i].y; }
+        const Point c = spolygon_centroid(s);
+        centroid2[0] = c.x;
+        centroid2[1] = c.y;
+        return (int)s.vertices.size();
+    } catch (...) {
+        return -1;
+    }
+}
+
+int or_ext_to_int(const float* ext, const float* pre, float* out) { return ext_to_int_transformation(ext, pre, out) ? 0 : -1; }
+int or_int_to_ext(const float* in, const float* pre, float* out) { return int_to_ext_transformation(in, pre, out) ? 0 : -1; }
+
 // Item built from already-known vertices, POI and surrogate (to check query parity on
 // shared inputs independently of the construction path).
 void* or_item_from_parts(const float* xy, uint32_t n, const float* meta9, const float* poles, uint32_t n_poles,
@@ -310,6 +345,23 @@ void* or_item_from_parts(const float* xy, uint32_t n, const float* meta9, const 
     return it;
 }
 void or_item_destroy(void* it) { delete (OrItem*)it; }
+// An item's shape: vertices (returns their count), meta = [area, diameter, bbox (4), poi (3)],
+// poles [n][3].
+int or_item_info(const void* it, float* out_xy, uint32_t xy_cap, float* meta9, float* poles, uint32_t poles_cap,
+                 uint32_t* n_poles) {
+    const SPolygon& s = ((const OrItem*)it)->shape;
+    for (size_t i = 0; i < s.vertices.size() && i < xy_cap; i++) { out_xy[2 * i] = s.vertices[i].x; out_xy[2 * i + 1] = s.vertices[i].y; }
+    const float m[9] = {s.area, s.diameter, s.bbox.x_min, s.bbox.y_min, s.bbox.x_max, s.bbox.y_max,
+                        s.poi.center.x, s.poi.center.y, s.poi.radius};
+    std::memcpy(meta9, m, sizeof m);
+    *n_poles = (uint32_t)s.surrogate.poles.size();
+    for (uint32_t i = 0; i < *n_poles && i < poles_cap; i++) {
+        poles[3 * i] = s.surrogate.poles[i].center.x;
+        poles[3 * i + 1] = s.surrogate.poles[i].center.y;
+        poles[3 * i + 2] = s.surrogate.poles[i].radius;
+    }
+    return (int)s.vertices.size();
+}
 
 // One batch of collision queries over `n` DTransformations (dt[i] = rotation, tx, ty) or, when
 // `mat` is non-null, over the 2x3 matrices mat[i] (m00 m01 m02 m10 m11 m12).
