@@ -89,7 +89,7 @@ def tune_and_time(plan, mats, status, args, stream):
     import torch
     log("autotuning the batched schedule")
     default_ms = timed(lambda: plan.detect(mats, status), 3, stream)
-    cfg, tuned_ms = plan.autotune(mats, status, reps=3)
+    cfg, tuned_ms = plan.autotune(mats, status, reps=5)
     log(f"schedule {cfg}: {tuned_ms:.3f} ms (default schedule {default_ms:.3f} ms)")
     for _ in range(args.warmup):
         plan.detect(mats, status)

@@ -2,8 +2,8 @@
 // in one launch sequence fed by device work queues.
 //   k_surrogate: work units of up to 64 queries; lanes = (query, ff pole/pier) pairs, each
 //                lane descends from the root (cd_engine.rs:204-223).  Collisions are written
-//                out; every unit with survivors appends one (set, first query, 64-bit mask)
-//                group to the survivor queue.
+//                out; the survivors of a unit are appended to the survivor queue as
+//                (set, first query, mask) groups of up to `chunk` queries each.
 //   k_poly:      survivor groups; each query is handled by G lanes (G = the item's vertex
 //                count rounded up to a power of two, 8..64), 64/G queries per wavefront:
 //                transformed bbox, virtual root, the item's edges descending from it with a
@@ -42,7 +42,7 @@ struct Unit {              // 16 B
     uint32_t set;
     uint32_t q0;           // first query, relative to the set
     uint32_t count;        // <= 64
-    uint32_t pad;
+    uint32_t group;        // queries per survivor group (16, 32 or 64)
 };
 
 struct Group {             // 16 B: surviving queries q0 + i for the set bits i of mask
@@ -150,10 +150,13 @@ __global__ __launch_bounds__(WAVE) void k_surrogate(const SetDesc* __restrict__ 
             if (h) status[base + lane] = JG_Q_COLLISION;
             else if (!(flags & JG_QUERY_POLY)) status[base + lane] = JG_Q_CLEAR;
         }
-        if (todo != 0ull && lane == 0) {
-            const uint32_t gi = atomicAdd(&counters[C_GROUPS], 1u);
-            groups[gi] = Group{un.set, un.q0, todo};
-        }
+        if (todo != 0ull && lane == 0)
+            for (uint32_t b = 0; b < cnt; b += un.group) {
+                const uint64_t sub = (todo >> b) & (un.group >= 64u ? ~0ull : ((1ull << un.group) - 1ull));
+                if (sub == 0ull) continue;
+                const uint32_t gi = atomicAdd(&counters[C_GROUPS], 1u);
+                groups[gi] = Group{un.set, un.q0 + b, sub};
+            }
         wave_sync();
     }
 }
@@ -370,8 +373,10 @@ jg_status plan_build_table(jg_plan* p) {
     }
     p->filtered = any_filter;
     p->kind = any_pier ? (any_pole ? KIND_MIXED : KIND_EDGE) : KIND_CIRCLE;
-    // work units: sets in decreasing estimated cost, chunks of `chunk` queries
-    const uint32_t chunk = p->cfg.chunk ? std::min<uint32_t>(p->cfg.chunk, 64u) : 64u;
+    // work units: sets in decreasing estimated cost, 64 queries each (the surrogate stage packs
+    // 128 pole entities per traversal); survivors leave in groups of `chunk` queries
+    const uint32_t chunk = p->cfg.chunk ? std::min<uint32_t>(p->cfg.chunk, 64u) : 32u;
+    const uint32_t group = chunk <= 16u ? 16u : chunk <= 32u ? 32u : 64u;
     std::vector<size_t> order(p->sets.size());
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
@@ -380,14 +385,15 @@ jg_status plan_build_table(jg_plan* p) {
     std::vector<Unit> units;
     for (size_t oi : order) {
         const jg_query_set& s = p->sets[oi];
-        for (uint64_t q = 0; q < s.count; q += chunk)
-            units.push_back(Unit{(uint32_t)oi, (uint32_t)q, (uint32_t)std::min<uint64_t>(chunk, s.count - q), 0u});
+        for (uint64_t q = 0; q < s.count; q += 64u)
+            units.push_back(Unit{(uint32_t)oi, (uint32_t)q, (uint32_t)std::min<uint64_t>(64u, s.count - q), group});
     }
     p->n_units = (uint32_t)units.size();
     const size_t o_units = (sizeof(SetDesc) * sd.size() + 255) & ~(size_t)255;
     const size_t o_ctr = (o_units + sizeof(Unit) * units.size() + 255) & ~(size_t)255;
     const size_t o_grp = o_ctr + 256;
-    const size_t total = o_grp + sizeof(Group) * (units.size() + 1);
+    const size_t max_groups = units.size() * (64u / group);
+    const size_t total = o_grp + sizeof(Group) * (max_groups + 1);
     std::vector<uint8_t> host(total, 0);
     std::memcpy(host.data(), sd.data(), sizeof(SetDesc) * sd.size());
     std::memcpy(host.data() + o_units, units.data(), sizeof(Unit) * units.size());
@@ -412,7 +418,7 @@ jg_status plan_build_table(jg_plan* p) {
     // one wavefront per workgroup, each taking one work unit (or survivor group) at a time
     const uint32_t want = p->cfg.workgroups ? p->cfg.workgroups : (uint32_t)cus * 16u;
     p->grid_s = std::max<uint32_t>(1u, std::min<uint32_t>(want, p->n_units));
-    p->grid_p = std::max<uint32_t>(1u, std::min<uint32_t>(want, p->n_units));
+    p->grid_p = std::max<uint32_t>(1u, (uint32_t)std::min<size_t>(want, max_groups));
     return JG_OK;
 }
 
@@ -591,24 +597,24 @@ extern "C" jg_status jg_plan_autotune(jg_plan* p, const float* d_m6, uint8_t* d_
     const std::vector<uint32_t> lanes = {0u, 8u, 16u, 32u, 64u};
     const std::vector<uint32_t> wgs = {4u * cus, 8u * cus, 16u * cus, 32u * cus};
     const std::vector<uint32_t> chunks = {16u, 32u, 64u};
-    for (int dim = 0; dim < 3 && st == JG_OK; dim++) {
-        const std::vector<uint32_t>& vals = dim == 0 ? lanes : dim == 1 ? wgs : chunks;
-        for (uint32_t v : vals) {
-            jg_plan_config c = cur;
-            uint32_t& field = dim == 0 ? c.lanes_per_query : dim == 1 ? c.workgroups : c.chunk;
-            const uint32_t was = dim == 0 ? cur.lanes_per_query : dim == 1 ? cur.workgroups : cur.chunk;
-            if (v == was) continue;
-            field = v;
-            p->cfg = c;
-            float ms = 0.0f;
-            st = plan_time(p, d_m6, d_status, reps, s, a, b, ms);
-            if (st != JG_OK) break;
-            if (ms < cur_ms) {
-                cur = c;
-                cur_ms = ms;
+    // the whole grid (60 schedules): timings are noisy enough that coordinate descent can stop
+    // in a poor corner
+    for (uint32_t lv : lanes)
+        for (uint32_t wv : wgs)
+            for (uint32_t cv : chunks) {
+                if (st != JG_OK) break;
+                jg_plan_config c = p->cfg;
+                c.lanes_per_query = lv;
+                c.workgroups = wv;
+                c.chunk = cv;
+                p->cfg = c;
+                float ms = 0.0f;
+                st = plan_time(p, d_m6, d_status, reps, s, a, b, ms);
+                if (st == JG_OK && ms < cur_ms) {
+                    cur = c;
+                    cur_ms = ms;
+                }
             }
-        }
-    }
     if (a) (void)hipEventDestroy(a);
     if (b) (void)hipEventDestroy(b);
     p->cfg = cur;

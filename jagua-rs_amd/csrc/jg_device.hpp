@@ -210,6 +210,9 @@ constexpr uint32_t WL_ENT = 128;          // entities per traversal
 constexpr uint32_t WL_NCAP = 512;         // node-visit queue entries per wave
 constexpr uint32_t WL_ECAP = 1024;        // edge-test queue entries per wave
 constexpr int WL_EDGE_BITS = 10;          // edge tests one leaf visit pushes: fewer than 2^WL_EDGE_BITS
+static_assert(64u * ((1u << WL_EDGE_BITS) - 1u) < 65536u, "a wave's edge pushes must fit 16 bits");
+constexpr uint32_t WL_RESERVE = 3u * (JG_MAX_QT_DEPTH + 1u);   // node slots a depth-first walk may need
+static_assert(WL_NCAP >= WL_ENT + WL_RESERVE + 3u, "node queue must hold the roots and the reserve");
 constexpr uint32_t WL_NODE_MASK = 0xffffffu;
 constexpr uint32_t WL_MAX_NODES = 1u << 24;
 constexpr uint32_t WL_CIRCLE = 0x80u;     // WorkList::ent_slot flag: the entity is a circle
@@ -308,6 +311,26 @@ __device__ __forceinline__ uint32_t wave_prefix(uint32_t x, uint32_t lane, uint3
     return off;
 }
 
+// Inclusive prefix sum of x over the 64 lanes of a wave with every lane active: DPP row shifts
+// within rows of 16 lanes, then the row broadcasts of lanes 15 and 31 (GFX9 DPP).
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, uint32_t lane) {
+    const uint32_t rl = lane & 15u;
+    uint32_t v = x;
+    uint32_t t = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x111, 0xf, 0xf, false);   // row_shr:1
+    if (rl >= 1u) v += t;
+    t = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x112, 0xf, 0xf, false);            // row_shr:2
+    if (rl >= 2u) v += t;
+    t = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x114, 0xf, 0xf, false);            // row_shr:4
+    if (rl >= 4u) v += t;
+    t = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x118, 0xf, 0xf, false);            // row_shr:8
+    if (rl >= 8u) v += t;
+    t = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x142, 0xf, 0xf, false);            // row_bcast:15
+    if ((lane & 31u) >= 16u) v += t;
+    t = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x143, 0xf, 0xf, false);            // row_bcast:31
+    if (lane >= 32u) v += t;
+    return v;
+}
+
 // Pushes one node visit per lane with want set onto a node queue of depth n (callers set
 // geom/ent_slot first); returns the new depth, the same in every lane.
 __device__ __forceinline__ uint32_t wave_push_roots(const WorkList& W, uint32_t n, bool want, uint32_t ent,
@@ -343,8 +366,17 @@ __device__ __forceinline__ void wave_traverse(const CdeView& c, const FilterView
             ne -= take;
             continue;
         }
-        // node step
-        const uint32_t take = nn < 64u ? nn : 64u;
+        // node step.  An item pushes at most 4 children, 3 more slots than it frees, so taking t
+        // items grows the queue by at most 3t: steps take no more than keeps WL_RESERVE slots
+        // free.  Within the reserve, single-item steps walk depth-first, which grows the queue by
+        // at most 3 per level below where the walk began; the reserve covers the deepest tree.
+        // The queue therefore never overflows and every step makes progress.
+        uint32_t take = 1u;
+        if (nn + 3u + WL_RESERVE <= WL_NCAP) {
+            const uint32_t room = (WL_NCAP - WL_RESERVE - nn) / 3u;
+            take = nn < 64u ? nn : 64u;
+            take = take < room ? take : room;
+        }
         const uint32_t base = nn - take;
         const bool have = lane < take;
         const uint32_t item = have ? W.nodes[nn - 1u - lane] : 0u;
@@ -374,9 +406,7 @@ __device__ __forceinline__ void wave_traverse(const CdeView& c, const FilterView
                         for (int k = 0; k < 4; k++)
                             m |= circle_box(g.x, g.y, g.z, box_quadrant(nd.bbox, k)) ? (1u << k) : 0u;
                     } else {
-                        float4 qs[4];
-                        for (int k = 0; k < 4; k++) qs[k] = box_quadrant(nd.bbox, k);
-                        m = edge_quadrants(g, nd.bbox, qs);
+                        m = edge_quadrants(g, nd.bbox);
                     }
                     s = -1;
                 } else if (s >= 0) {
@@ -400,23 +430,19 @@ __device__ __forceinline__ void wave_traverse(const CdeView& c, const FilterView
             if (h) W.hit[slot] = 1u;
             n_edges = 0;
         }
-        uint32_t n_tot, e_tot;
-        const uint32_t n_off = wave_prefix<3>((uint32_t)__builtin_popcount(m), lane, n_tot);
-        const uint32_t e_off = wave_prefix<WL_EDGE_BITS>(n_edges, lane, e_tot);
+        // one scan for both pushes: child count in the low 16 bits, edge count (< 2^WL_EDGE_BITS)
+        // in the high 16; neither sum over 64 lanes carries out of its half
+        const uint32_t packed = (uint32_t)__builtin_popcount(m) | (n_edges << 16);
+        const uint32_t incl = wave_incl_scan(packed, lane);
+        const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+        const uint32_t excl = incl - packed;
+        const uint32_t n_off = excl & 0xffffu, e_off = excl >> 16;
+        const uint32_t n_tot = tot & 0xffffu, e_tot = tot >> 16;
         wave_sync();                              // every lane has read its item before pushes overwrite
-        if (base + n_tot <= WL_NCAP) {
-            uint32_t at = base + n_off;
+        {
+            uint32_t at = base + n_off;                // base + n_tot <= WL_NCAP by the choice of take
             for (uint32_t r = m; r; r &= r - 1u) W.nodes[at++] = (item & ~WL_NODE_MASK) | (uint32_t)(fc + __builtin_ctz(r));
             nn = base + n_tot;
-        } else {
-            // node queue full: these lanes finish their children's subtrees on their own
-            bool h = false;
-            for (uint32_t r = m; r && !h; r &= r - 1u)
-                h = descend(c, fc + __builtin_ctz(r), [&](int nd2, int& ffc) -> uint32_t {
-                    return visit_node<FILT>(c, nd2, circle, g, f, q, ffc);
-                });
-            if (h) W.hit[slot] = 1u;
-            nn = base;
         }
         if (ne + e_tot <= WL_ECAP) {
             uint32_t at = ne + e_off;

@@ -140,6 +140,23 @@ JG_HD void half_update(uint32_t& st, float4 e, float x3, float y3, float x4, flo
 
 // qt_traits.rs:19-106  Edge::collides_with_quadrants; r = node box, q[k] = its children.
 // Returns a 4-bit mask of colliding quadrants.
+JG_HD uint32_t edge_quadrants_sides(float4 e, float4 r, uint32_t st);
+
+// The same with the quadrant boxes derived from r on the fly (box_quadrant), keeping fewer
+// values live in device code.
+JG_HD uint32_t edge_quadrants(float4 e, float4 r) {
+    float ex0 = mn(e.x, e.z), ex1 = mx(e.x, e.z), ey0 = mn(e.y, e.w), ey1 = mx(e.y, e.w);
+    uint32_t st = 0;
+    for (int k = 0; k < 4; k++) {
+        const float4 b = box_quadrant(r, k);
+        bool no = mx(ex0, b.x) > mn(ex1, b.z) || mx(ey0, b.y) > mn(ey1, b.w);
+        if (no) st |= 1u << k;
+        else if (box_has_point(b, e.x, e.y) || box_has_point(b, e.z, e.w)) st |= (1u << k) | (1u << (k + 4));
+    }
+    if ((st & 0xfu) == 0xfu) return st >> 4;
+    return edge_quadrants_sides(e, r, st);
+}
+
 JG_HD uint32_t edge_quadrants(float4 e, float4 r, const float4* q) {
     float ex0 = mn(e.x, e.z), ex1 = mx(e.x, e.z), ey0 = mn(e.y, e.w), ey1 = mx(e.y, e.w);
     uint32_t st = 0;
@@ -150,6 +167,12 @@ JG_HD uint32_t edge_quadrants(float4 e, float4 r, const float4* q) {
         else if (box_has_point(b, e.x, e.y) || box_has_point(b, e.z, e.w)) st |= (1u << k) | (1u << (k + 4));
     }
     if ((st & 0xfu) == 0xfu) return st >> 4;
+    return edge_quadrants_sides(e, r, st);
+}
+
+// The side and bisector half-intersections of collides_with_quadrants for the quadrants not
+// yet determined in st.
+JG_HD uint32_t edge_quadrants_sides(float4 e, float4 r, uint32_t st) {
     float cx = (r.x + r.z) / 2.0f, cy = (r.y + r.w) / 2.0f;
     // sides (rect.rs:164-184): top (x_max,y_max)->(x_min,y_max), left (x_min,y_max)->(x_min,y_min),
     // bottom (x_min,y_min)->(x_max,y_min), right (x_max,y_min)->(x_max,y_max)
