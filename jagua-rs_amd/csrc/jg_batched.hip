@@ -140,7 +140,7 @@ __global__ __launch_bounds__(WAVE) void k_surrogate(const SetDesc* __restrict__ 
                     }
                     n = wave_push_roots(W, n, want, o, 0u, lane);
                 }
-                wave_traverse<FILT, KIND>(c, f, W, n, lane);
+                wave_traverse<FILT, KIND, 2>(c, f, W, n, lane);
             }
             hit_mask = __ballot(W.hit[lane] != 0u) & valid;
         }

@@ -343,25 +343,40 @@ __device__ __forceinline__ uint32_t wave_push_roots(const WorkList& W, uint32_t 
 
 // Runs the wave's nn pending node visits (and the edge tests they create) to completion;
 // W.hit[slot] is set for every query slot one of whose entities collides.  Callers push at
-// most WL_ENT roots (WL_ENT <= WL_NCAP).
-template <bool FILT, int KIND>
+// most WL_ENT roots (WL_ENT <= WL_NCAP).  Each lane handles IPL items per step (node visits or
+// edge tests): their memory reads are issued together, so a step waits for one round trip
+// instead of IPL.
+template <bool FILT, int KIND, int IPL = 1>
 __device__ __forceinline__ void wave_traverse(const CdeView& c, const FilterView& f, const WorkList& W, uint32_t nn,
                                               uint32_t lane) {
+    static_assert(IPL == 1 || IPL == 2, "one or two items per lane");
+    // edge tests one item may push; the per-lane sum over IPL items stays below 2^16 / 64
+    constexpr uint32_t EDGE_LIMIT = IPL == 1 ? (1u << WL_EDGE_BITS) : (1u << (WL_EDGE_BITS - 1));
+    static_assert(64u * IPL * (EDGE_LIMIT - 1u) < 65536u, "a wave's edge pushes must fit 16 bits");
+    constexpr uint32_t STEP = 64u * IPL;
     uint32_t ne = 0;
     while (nn + ne != 0u) {
         wave_sync();                              // pushes and hit flags of the last step are visible
-        if (ne >= 64u || nn == 0u) {
-            // edge step
-            const uint32_t take = ne < 64u ? ne : 64u;
-            if (lane < take) {
-                const uint32_t at = ne - 1u - lane;
-                const uint32_t o = W.edge_ent[at];
-                const uint32_t es = W.ent_slot[o];
+        if (ne >= STEP || nn == 0u) {
+            // edge step: both tests' edges are read before either is evaluated
+            const uint32_t take = ne < STEP ? ne : STEP;
+            uint32_t eo[IPL];
+            float4 hedge[IPL];
+#pragma unroll
+            for (int r = 0; r < IPL; r++) {
+                const uint32_t idx = lane + 64u * (uint32_t)r;
+                const uint32_t at = idx < take ? ne - 1u - idx : 0u;
+                eo[r] = W.edge_ent[at];
+                hedge[r] = c.edges[idx < take ? W.edges[at] : 0u];
+            }
+#pragma unroll
+            for (int r = 0; r < IPL; r++) {
+                if (lane + 64u * (uint32_t)r >= take) continue;
+                const uint32_t es = W.ent_slot[eo[r]];
                 const uint32_t slot = es & 63u;
-                if (!W.hit[slot]) {
-                    const bool circle = KIND == KIND_CIRCLE || (KIND == KIND_MIXED && (es & WL_CIRCLE) != 0u);
-                    if (edge_hit(circle, W.geom[o], c.edges[W.edges[at]])) W.hit[slot] = 1u;
-                }
+                if (W.hit[slot]) continue;
+                const bool circle = KIND == KIND_CIRCLE || (KIND == KIND_MIXED && (es & WL_CIRCLE) != 0u);
+                if (edge_hit(circle, W.geom[eo[r]], hedge[r])) W.hit[slot] = 1u;
             }
             ne -= take;
             continue;
@@ -374,98 +389,122 @@ __device__ __forceinline__ void wave_traverse(const CdeView& c, const FilterView
         uint32_t take = 1u;
         if (nn + 3u + WL_RESERVE <= WL_NCAP) {
             const uint32_t room = (WL_NCAP - WL_RESERVE - nn) / 3u;
-            take = nn < 64u ? nn : 64u;
+            take = nn < STEP ? nn : STEP;
             take = take < room ? take : room;
         }
         const uint32_t base = nn - take;
-        const bool have = lane < take;
-        const uint32_t item = have ? W.nodes[nn - 1u - lane] : 0u;
-        const uint32_t o = item >> 24;
-        uint32_t m = 0, n_edges = 0;
-        int fc = 0, s = -1, he = 0;
-        uint32_t slot = 0;
-        bool circle = KIND == KIND_CIRCLE;
-        float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        uint64_t q = 0;
-        if (have) {
-            const uint32_t es = W.ent_slot[o];
-            slot = es & 63u;
-            if (!W.hit[slot]) {
-                g = W.geom[o];
-                if (KIND == KIND_MIXED) circle = (es & WL_CIRCLE) != 0u;
-                if (FILT) q = W.qidx[slot];
-                const DevNode nd = c.nodes[item & WL_NODE_MASK];
-                bool entire = false;
-                s = strongest_t<FILT>(c, nd, f, q, entire);
-                if (s >= 0 && entire) {
-                    W.hit[slot] = 1u;
-                    s = -1;
-                } else if (s >= 0 && nd.first_child >= 0) {
-                    fc = nd.first_child;
-                    if (circle) {
-                        for (int k = 0; k < 4; k++)
-                            m |= circle_box(g.x, g.y, g.z, box_quadrant(nd.bbox, k)) ? (1u << k) : 0u;
-                    } else {
-                        m = edge_quadrants(g, nd.bbox);
-                    }
-                    s = -1;
-                } else if (s >= 0) {
-                    // leaf: count the edge tests of its relevant Partial hazards
-                    he = (int)(nd.haz_begin + (nd.haz_info & COUNT_MASK));
-                    for (int i = s; i < he; i++) {
-                        const DevEntry e = c.ent[i];
-                        if (!irrel<FILT>(f, q, e.slot_flags & SLOT_MASK) && ff_pass(e, circle, g)) n_edges += e.edge_count;
+        uint32_t item[IPL], m[IPL], n_edges[IPL], slot[IPL];
+        int fc[IPL], s[IPL], he[IPL];
+        bool circle[IPL];
+        float4 g[IPL];
+        uint64_t q[IPL];
+        DevNode nd[IPL];
+#pragma unroll
+        for (int r = 0; r < IPL; r++) {                // the node records, all in flight together
+            const uint32_t idx = lane + 64u * (uint32_t)r;
+            item[r] = idx < take ? W.nodes[nn - 1u - idx] : 0u;
+            nd[r] = c.nodes[item[r] & WL_NODE_MASK];
+        }
+        uint32_t packed = 0;
+#pragma unroll
+        for (int r = 0; r < IPL; r++) {
+            const uint32_t o = item[r] >> 24;
+            m[r] = 0;
+            n_edges[r] = 0;
+            fc[r] = 0;
+            s[r] = -1;
+            he[r] = 0;
+            slot[r] = 0;
+            circle[r] = KIND == KIND_CIRCLE;
+            g[r] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            q[r] = 0;
+            if (lane + 64u * (uint32_t)r < take) {
+                const uint32_t es = W.ent_slot[o];
+                slot[r] = es & 63u;
+                if (!W.hit[slot[r]]) {
+                    g[r] = W.geom[o];
+                    if (KIND == KIND_MIXED) circle[r] = (es & WL_CIRCLE) != 0u;
+                    if (FILT) q[r] = W.qidx[slot[r]];
+                    bool entire = false;
+                    s[r] = strongest_t<FILT>(c, nd[r], f, q[r], entire);
+                    if (s[r] >= 0 && entire) {
+                        W.hit[slot[r]] = 1u;
+                        s[r] = -1;
+                    } else if (s[r] >= 0 && nd[r].first_child >= 0) {
+                        fc[r] = nd[r].first_child;
+                        if (circle[r]) {
+                            for (int k = 0; k < 4; k++)
+                                m[r] |= circle_box(g[r].x, g[r].y, g[r].z, box_quadrant(nd[r].bbox, k)) ? (1u << k) : 0u;
+                        } else {
+                            m[r] = edge_quadrants(g[r], nd[r].bbox);
+                        }
+                        s[r] = -1;
+                    } else if (s[r] >= 0) {
+                        // leaf: count the edge tests of its relevant Partial hazards
+                        he[r] = (int)(nd[r].haz_begin + (nd[r].haz_info & COUNT_MASK));
+                        for (int i = s[r]; i < he[r]; i++) {
+                            const DevEntry e = c.ent[i];
+                            if (!irrel<FILT>(f, q[r], e.slot_flags & SLOT_MASK) && ff_pass(e, circle[r], g[r]))
+                                n_edges[r] += e.edge_count;
+                        }
                     }
                 }
             }
-        }
-        if (n_edges >= (1u << WL_EDGE_BITS)) {
-            // more edge tests than one push carries: this lane tests its leaf's edges itself
-            bool h = false;
-            for (int i = s; i < he && !h; i++) {
-                const DevEntry e = c.ent[i];
-                if (irrel<FILT>(f, q, e.slot_flags & SLOT_MASK) || !ff_pass(e, circle, g)) continue;
-                for (uint32_t j = 0; j < e.edge_count && !h; j++) h = edge_hit(circle, g, c.edges[e.edge_begin + j]);
+            if (n_edges[r] >= EDGE_LIMIT) {
+                // more edge tests than one push carries: this lane tests the leaf's edges itself
+                bool h = false;
+                for (int i = s[r]; i < he[r] && !h; i++) {
+                    const DevEntry e = c.ent[i];
+                    if (irrel<FILT>(f, q[r], e.slot_flags & SLOT_MASK) || !ff_pass(e, circle[r], g[r])) continue;
+                    for (uint32_t j = 0; j < e.edge_count && !h; j++) h = edge_hit(circle[r], g[r], c.edges[e.edge_begin + j]);
+                }
+                if (h) W.hit[slot[r]] = 1u;
+                n_edges[r] = 0;
             }
-            if (h) W.hit[slot] = 1u;
-            n_edges = 0;
+            // one scan for both pushes: child counts in the low 16 bits, edge counts in the high
+            // 16; neither sum over the wave carries out of its half
+            packed += (uint32_t)__builtin_popcount(m[r]) | (n_edges[r] << 16);
         }
-        // one scan for both pushes: child count in the low 16 bits, edge count (< 2^WL_EDGE_BITS)
-        // in the high 16; neither sum over 64 lanes carries out of its half
-        const uint32_t packed = (uint32_t)__builtin_popcount(m) | (n_edges << 16);
         const uint32_t incl = wave_incl_scan(packed, lane);
         const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
         const uint32_t excl = incl - packed;
-        const uint32_t n_off = excl & 0xffffu, e_off = excl >> 16;
         const uint32_t n_tot = tot & 0xffffu, e_tot = tot >> 16;
-        wave_sync();                              // every lane has read its item before pushes overwrite
+        wave_sync();                              // every lane has read its items before pushes overwrite
         {
-            uint32_t at = base + n_off;                // base + n_tot <= WL_NCAP by the choice of take
-            for (uint32_t r = m; r; r &= r - 1u) W.nodes[at++] = (item & ~WL_NODE_MASK) | (uint32_t)(fc + __builtin_ctz(r));
+            uint32_t at = base + (excl & 0xffffu);     // base + n_tot <= WL_NCAP by the choice of take
+#pragma unroll
+            for (int r = 0; r < IPL; r++)
+                for (uint32_t b = m[r]; b; b &= b - 1u)
+                    W.nodes[at++] = (item[r] & ~WL_NODE_MASK) | (uint32_t)(fc[r] + __builtin_ctz(b));
             nn = base + n_tot;
         }
-        if (ne + e_tot <= WL_ECAP) {
-            uint32_t at = ne + e_off;
-            if (n_edges)
-                for (int i = s; i < he; i++) {
+        const bool fits = ne + e_tot <= WL_ECAP;
+        uint32_t at = ne + (excl >> 16);
+#pragma unroll
+        for (int r = 0; r < IPL; r++) {
+            if (!n_edges[r]) continue;
+            if (fits) {
+                const uint32_t o = item[r] >> 24;
+                for (int i = s[r]; i < he[r]; i++) {
                     const DevEntry e = c.ent[i];
-                    if (irrel<FILT>(f, q, e.slot_flags & SLOT_MASK) || !ff_pass(e, circle, g)) continue;
+                    if (irrel<FILT>(f, q[r], e.slot_flags & SLOT_MASK) || !ff_pass(e, circle[r], g[r])) continue;
                     for (uint32_t j = 0; j < e.edge_count; j++, at++) {
                         W.edges[at] = e.edge_begin + j;
                         W.edge_ent[at] = (uint8_t)o;
                     }
                 }
-            ne += e_tot;
-        } else if (n_edges) {
-            // edge queue full: this lane tests its leaf's edges itself
-            bool h = false;
-            for (int i = s; i < he && !h; i++) {
-                const DevEntry e = c.ent[i];
-                if (irrel<FILT>(f, q, e.slot_flags & SLOT_MASK) || !ff_pass(e, circle, g)) continue;
-                for (uint32_t j = 0; j < e.edge_count && !h; j++) h = edge_hit(circle, g, c.edges[e.edge_begin + j]);
+            } else {
+                // edge queue full: this lane tests the leaf's edges itself
+                bool h = false;
+                for (int i = s[r]; i < he[r] && !h; i++) {
+                    const DevEntry e = c.ent[i];
+                    if (irrel<FILT>(f, q[r], e.slot_flags & SLOT_MASK) || !ff_pass(e, circle[r], g[r])) continue;
+                    for (uint32_t j = 0; j < e.edge_count && !h; j++) h = edge_hit(circle[r], g[r], c.edges[e.edge_begin + j]);
+                }
+                if (h) W.hit[slot[r]] = 1u;
             }
-            if (h) W.hit[slot] = 1u;
         }
+        if (fits) ne += e_tot;
     }
     wave_sync();
 }
