@@ -152,11 +152,29 @@ def run_c2(args):
         plan.detect_graph(mats_all, status_graph, replays=1)
         ms_graph = timed(lambda: plan.detect_graph(mats_all, status_graph, replays=1), args.steps, stream)
         graph_ok = bool(torch.equal(status_graph, status))
+        # hazard sets of the same step (collect_surrogate_collisions U collect_poly_collisions,
+        # ci_bench.rs:56-69): count, hash and the first 8 keys per query, one launch per item
+        cnt = torch.empty(total, dtype=torch.int32, device=dev)
+        hsh = torch.empty(total, dtype=torch.int64, device=dev)
+        keys = torch.empty((total, 8), dtype=torch.int32, device=dev)
+
+        def collect_step():
+            for i, it in enumerate(items):
+                a, b = int(offs[i]), int(offs[i + 1])
+                cde.collect_batch(it, mats_all[a:b], capacity=8, out=(cnt[a:b], hsh[a:b], keys[a:b]))
+
+        collect_step()
+        ms_collect = timed(collect_step, 3, stream)
+        sets_agree = bool(torch.equal(cnt > 0, status.to(torch.int32) > 0))
+        log(f"collect: {ms_collect:.3f} ms/step ({total / (ms_collect / 1e3) / 1e6:.1f} M queries/s); "
+            f"non-empty set <=> collision on every query: {sets_agree}")
         result["forms"] = {
             "plain_queries_per_s": total / (ms_plain / 1e3), "plain_ms_per_step": ms_plain,
             "batched_queries_per_s": qps, "graph_queries_per_s": total / (ms_graph / 1e3),
             "graph_ms_per_step": ms_graph, "batched_over_plain": ms_plain / ms,
-            "plain_matches_batched": plain_ok, "graph_matches_batched": graph_ok}
+            "plain_matches_batched": plain_ok, "graph_matches_batched": graph_ok,
+            "collect_queries_per_s": total / (ms_collect / 1e3), "collect_ms_per_step": ms_collect,
+            "collect_sets_match_verdicts": sets_agree}
         log(f"plain: {ms_plain:.3f} ms/step; graph: {ms_graph:.3f} ms/step; same verdicts: {plain_ok and graph_ok}")
         result["overlap"] = overlap_c4(stream)
         result["search"] = search_c2(cde, items, stream)

@@ -9,8 +9,9 @@
 //                transformed bbox, virtual root, the item's edges descending from it with a
 //                group-wide early exit, then containment over the virtual root's hazards
 //                (cd_engine.rs:160-197).
-// Splitting the two stages keeps the light surrogate kernel at high occupancy and lets the
-// polygon stage pack several small items per wavefront.  Units are ordered by estimated
+// k_fused runs both stages of a work unit in one wave (the default schedule); the two-kernel
+// form (k_surrogate, then k_poly over the survivor queue) is the alternative the autotuner
+// weighs.  Units are ordered by estimated
 // cost (largest items first) so the tail stays short.
 #include <hip/hip_runtime.h>
 
@@ -81,6 +82,70 @@ __global__ void k_reset_counters(uint32_t* counters) {
     if (threadIdx.x < 4) counters[threadIdx.x] = 0u;
 }
 
+// The surrogate stage of one work unit (cd_engine.rs:204-223 for each of its queries): writes
+// the verdicts it settles and returns the mask of queries (bit i = query un.q0 + i) that go on
+// to the polygon stage.
+template <bool FILT, int KIND, int IPL>
+__device__ __forceinline__ uint64_t surrogate_unit(const SetDesc& sd, const Unit& un, const WorkList& W,
+                                                   const float* __restrict__ m6, uint8_t* __restrict__ status,
+                                                   uint32_t lane) {
+    const CdeView c = sd.cde;
+    const ItemView it = sd.item;
+    const FilterView f = sd.filter;
+    const uint32_t flags = sd.flags;
+    const uint64_t base = sd.first + un.q0;
+    const uint32_t cnt = un.count;
+    const uint64_t valid = cnt == 64 ? ~0ull : ((1ull << cnt) - 1ull);
+
+    uint64_t hit_mask = 0;
+    const uint32_t n_ent = (flags & JG_QUERY_SURROGATE) ? it.n_poles + it.n_piers : 0u;
+    if (n_ent > 0) {
+        // every (query, ff pole / pier) of the unit descends from the root
+        // (cd_engine.rs:204-223), WL_ENT entities per traversal
+        W.hit[lane] = 0u;
+        if (FILT) W.qidx[lane] = un.q0 + lane;
+        const uint32_t total = cnt * n_ent;
+        for (uint32_t e0 = 0; e0 < total; e0 += WL_ENT) {
+            wave_sync();
+            uint32_t n = 0;
+            for (uint32_t j = 0; j < WL_ENT; j += WAVE) {
+                const uint32_t o = j + lane;
+                const uint32_t e = e0 + o;
+                bool want = false;
+                if (e < total) {
+                    const uint32_t qi = e / n_ent, k = e - qi * n_ent;
+                    if (!W.hit[qi]) {
+                        const Mat m = load_mat(m6, base + qi);
+                        const float4 p = it.ff[k];
+                        const bool circle = k < it.n_poles;
+                        float4 gm;
+                        if (circle) {
+                            const float2 cc = xform(m, p.x, p.y);
+                            gm = make_float4(cc.x, cc.y, p.z, 0.0f);
+                        } else {
+                            const float2 a = xform(m, p.x, p.y), bb = xform(m, p.z, p.w);
+                            gm = make_float4(a.x, a.y, bb.x, bb.y);
+                        }
+                        W.geom[o] = gm;
+                        W.ent_slot[o] = (uint8_t)(qi | (circle ? WL_CIRCLE : 0u));
+                        want = true;
+                    }
+                }
+                n = wave_push_roots(W, n, want, o, 0u, lane);
+            }
+            wave_traverse<FILT, KIND, IPL>(c, f, W, n, lane);
+        }
+        hit_mask = __ballot(W.hit[lane] != 0u) & valid;
+    }
+    const uint64_t todo = (flags & JG_QUERY_POLY) ? (valid & ~hit_mask) : 0ull;
+    if (lane < cnt) {
+        const bool h = (hit_mask >> lane) & 1ull;
+        if (h) status[base + lane] = JG_Q_COLLISION;
+        else if (!(flags & JG_QUERY_POLY)) status[base + lane] = JG_Q_CLEAR;
+    }
+    return todo;
+}
+
 template <bool FILT, int KIND>
 __global__ __launch_bounds__(WAVE) void k_surrogate(const SetDesc* __restrict__ sets, const Unit* __restrict__ units,
                                                    uint32_t n_units, uint32_t* __restrict__ counters,
@@ -95,69 +160,115 @@ __global__ __launch_bounds__(WAVE) void k_surrogate(const SetDesc* __restrict__ 
         u = wave_bcast(u);
         if (u >= n_units) return;
         const Unit un = units[u];
-        const SetDesc& sd = sets[un.set];
-        const CdeView c = sd.cde;
-        const ItemView it = sd.item;
-        const FilterView f = sd.filter;
-        const uint32_t flags = sd.flags;
-        const uint64_t base = sd.first + un.q0;
-        const uint32_t cnt = un.count;
-        const uint64_t valid = cnt == 64 ? ~0ull : ((1ull << cnt) - 1ull);
-
-        uint64_t hit_mask = 0;
-        const uint32_t n_ent = (flags & JG_QUERY_SURROGATE) ? it.n_poles + it.n_piers : 0u;
-        if (n_ent > 0) {
-            // every (query, ff pole / pier) of the unit descends from the root
-            // (cd_engine.rs:204-223), WL_ENT entities per traversal
-            W.hit[lane] = 0u;
-            if (FILT) W.qidx[lane] = un.q0 + lane;
-            const uint32_t total = cnt * n_ent;
-            for (uint32_t e0 = 0; e0 < total; e0 += WL_ENT) {
-                wave_sync();
-                uint32_t n = 0;
-                for (uint32_t j = 0; j < WL_ENT; j += WAVE) {
-                    const uint32_t o = j + lane;
-                    const uint32_t e = e0 + o;
-                    bool want = false;
-                    if (e < total) {
-                        const uint32_t qi = e / n_ent, k = e - qi * n_ent;
-                        if (!W.hit[qi]) {
-                            const Mat m = load_mat(m6, base + qi);
-                            const float4 p = it.ff[k];
-                            const bool circle = k < it.n_poles;
-                            float4 gm;
-                            if (circle) {
-                                const float2 cc = xform(m, p.x, p.y);
-                                gm = make_float4(cc.x, cc.y, p.z, 0.0f);
-                            } else {
-                                const float2 a = xform(m, p.x, p.y), bb = xform(m, p.z, p.w);
-                                gm = make_float4(a.x, a.y, bb.x, bb.y);
-                            }
-                            W.geom[o] = gm;
-                            W.ent_slot[o] = (uint8_t)(qi | (circle ? WL_CIRCLE : 0u));
-                            want = true;
-                        }
-                    }
-                    n = wave_push_roots(W, n, want, o, 0u, lane);
-                }
-                wave_traverse<FILT, KIND, 2>(c, f, W, n, lane);
-            }
-            hit_mask = __ballot(W.hit[lane] != 0u) & valid;
-        }
-        const uint64_t todo = (flags & JG_QUERY_POLY) ? (valid & ~hit_mask) : 0ull;
-        if (lane < cnt) {
-            const bool h = (hit_mask >> lane) & 1ull;
-            if (h) status[base + lane] = JG_Q_COLLISION;
-            else if (!(flags & JG_QUERY_POLY)) status[base + lane] = JG_Q_CLEAR;
-        }
+        const uint64_t todo = surrogate_unit<FILT, KIND, 2>(sets[un.set], un, W, m6, status, lane);
         if (todo != 0ull && lane == 0)
-            for (uint32_t b = 0; b < cnt; b += un.group) {
+            for (uint32_t b = 0; b < un.count; b += un.group) {
                 const uint64_t sub = (todo >> b) & (un.group >= 64u ? ~0ull : ((1ull << un.group) - 1ull));
                 if (sub == 0ull) continue;
                 const uint32_t gi = atomicAdd(&counters[C_GROUPS], 1u);
                 groups[gi] = Group{un.set, un.q0 + b, sub};
             }
         wave_sync();
+    }
+}
+
+// The polygon stage (cd_engine.rs:160-197) of the queries q0 + i for the set bits i of mask,
+// all of one set: 64/G queries per pass, G lanes each.
+template <bool FILT>
+__device__ __forceinline__ void poly_queries(const SetDesc& sd, uint32_t q0, uint64_t mask, const WorkList& W,
+                                             const float* __restrict__ m6, uint8_t* __restrict__ status,
+                                             uint32_t lane) {
+    const CdeView c = sd.cde;
+    const ItemView it = sd.item;
+    const FilterView f = sd.filter;
+    const uint32_t G = sd.lanes;
+    const uint32_t per = 64u / G;                     // queries per pass
+    const uint32_t slot = lane / G, sub = lane % G;
+    const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+    const float4 root_bb = c.nodes[0].bbox;
+    const uint32_t nv = it.n_v;
+    uint64_t todo = mask;
+    while (todo) {
+        // this lane's query: the slot-th remaining survivor
+        uint64_t t = todo;
+        for (uint32_t s = 0; s < slot && t; s++) t &= t - 1;
+        const bool active = t != 0ull;
+        const uint32_t qi = active ? (uint32_t)__builtin_ctzll(t) : 0u;
+        for (uint32_t s = 0; s < per && todo; s++) todo &= todo - 1;
+        const uint64_t q = (uint64_t)q0 + qi;
+        Mat m{};
+        if (active) m = load_mat(m6, sd.first + q);
+        // transformed bbox (simple_polygon.rs:122-133), reduced over the group's lanes
+        float4 bb = make_float4(3.40282347e+38f, 3.40282347e+38f, -3.40282347e+38f, -3.40282347e+38f);
+        if (active)
+            for (uint32_t i = sub; i < nv; i += G) {
+                const float2 p = xform(m, it.v[i].x, it.v[i].y);
+                bb.x = mn(bb.x, p.x); bb.y = mn(bb.y, p.y); bb.z = mx(bb.z, p.x); bb.w = mx(bb.w, p.y);
+            }
+        for (uint32_t o = G >> 1; o > 0; o >>= 1) {
+            bb.x = fminf(bb.x, __shfl_xor(bb.x, (int)o, 64));
+            bb.y = fminf(bb.y, __shfl_xor(bb.y, (int)o, 64));
+            bb.z = fmaxf(bb.z, __shfl_xor(bb.z, (int)o, 64));
+            bb.w = fmaxf(bb.w, __shfl_xor(bb.w, (int)o, 64));
+        }
+        uint8_t res = JG_Q_CLEAR;
+        bool done = !active;
+        if (active && box_relation(root_bb, bb) != REL_SURROUNDING) {
+            res = JG_Q_COLLISION;
+            done = true;
+        }
+        const int vr = active ? virtual_root(c, bb) : 0;
+        // edges of the pass's unfinished queries, descending from their virtual roots
+        // (cd_engine.rs:170-175); a query slot is marked hit as soon as one edge collides
+        wave_sync();
+        if (sub == 0) {
+            W.hit[slot] = done ? 1u : 0u;
+            if (FILT) W.qidx[slot] = (uint32_t)q;
+        }
+        // round R: lane (slot, sub) sets up edges sub + G (2R + j), j < WL_ENT / 64
+        constexpr uint32_t EPL = WL_ENT / WAVE;           // entities per lane per round
+        for (uint32_t i0 = 0; i0 < nv; i0 += EPL * G) {
+            wave_sync();
+            if (__ballot(active && W.hit[slot] == 0u) == 0ull) break;
+            uint32_t n = 0;
+            for (uint32_t j = 0; j < EPL; j++) {
+                const uint32_t i = i0 + j * G + sub;
+                const uint32_t o = j * WAVE + lane;
+                const bool want = active && i < nv && W.hit[slot] == 0u;
+                if (want) {
+                    const uint32_t k = i + 1 == nv ? 0 : i + 1;
+                    const float2 a = xform(m, it.v[i].x, it.v[i].y);
+                    const float2 b2 = xform(m, it.v[k].x, it.v[k].y);
+                    W.geom[o] = make_float4(a.x, a.y, b2.x, b2.y);
+                    W.ent_slot[o] = (uint8_t)slot;
+                }
+                n = wave_push_roots(W, n, want, o, (uint32_t)vr, lane);
+            }
+            wave_traverse<FILT, KIND_EDGE>(c, f, W, n, lane);
+        }
+        // containment over the virtual root's hazards (cd_engine.rs:176-193)
+        wave_sync();
+        done = W.hit[slot] != 0u || done;
+        if (active && W.hit[slot] != 0u && res == JG_Q_CLEAR) res = JG_Q_COLLISION;
+        bool hit = false;
+        if (!done) {
+            const DevNode vn = c.nodes[vr];
+            const uint32_t cntv = vn.haz_info & COUNT_MASK;
+            if (cntv > 0 && (vn.haz_info & ENTIRE_BIT)) {
+                res = JG_Q_PANIC_FILTERED_ENTIRE;
+                done = true;
+            } else {
+                const float2 qpoi = xform(m, it.poi.x, it.poi.y);
+                for (uint32_t i = sub; i < cntv; i += G) {
+                    const uint32_t slot_h = c.ent[vn.haz_begin + i].slot_flags & SLOT_MASK;
+                    if (irrel<FILT>(f, q, slot_h)) continue;
+                    if (containment(c, slot_h, it, m, bb, qpoi)) { hit = true; break; }
+                }
+            }
+        }
+        const uint64_t bal = ballot(hit);
+        if (!done) res = ((bal >> (slot * G)) & gmask) ? JG_Q_COLLISION : JG_Q_CLEAR;
+        if (active && sub == 0) status[sd.first + q] = res;
     }
 }
 
@@ -175,99 +286,33 @@ __global__ __launch_bounds__(WAVE) void k_poly(const SetDesc* __restrict__ sets,
         gi = wave_bcast(gi);
         if (gi >= n_groups) return;
         const Group gr = groups[gi];
-        const SetDesc& sd = sets[gr.set];
-        const CdeView c = sd.cde;
-        const ItemView it = sd.item;
-        const FilterView f = sd.filter;
-        const uint32_t G = sd.lanes;
-        const uint32_t per = 64u / G;                     // queries per pass
-        const uint32_t slot = lane / G, sub = lane % G;
-        const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
-        const float4 root_bb = c.nodes[0].bbox;
-        const uint32_t nv = it.n_v;
-        uint64_t todo = gr.mask;
-        while (todo) {
-            // this lane's query: the slot-th remaining survivor
-            uint64_t t = todo;
-            for (uint32_t s = 0; s < slot && t; s++) t &= t - 1;
-            const bool active = t != 0ull;
-            const uint32_t qi = active ? (uint32_t)__builtin_ctzll(t) : 0u;
-            for (uint32_t s = 0; s < per && todo; s++) todo &= todo - 1;
-            const uint64_t q = (uint64_t)gr.q0 + qi;
-            Mat m{};
-            if (active) m = load_mat(m6, sd.first + q);
-            // transformed bbox (simple_polygon.rs:122-133), reduced over the group's lanes
-            float4 bb = make_float4(3.40282347e+38f, 3.40282347e+38f, -3.40282347e+38f, -3.40282347e+38f);
-            if (active)
-                for (uint32_t i = sub; i < nv; i += G) {
-                    const float2 p = xform(m, it.v[i].x, it.v[i].y);
-                    bb.x = mn(bb.x, p.x); bb.y = mn(bb.y, p.y); bb.z = mx(bb.z, p.x); bb.w = mx(bb.w, p.y);
-                }
-            for (uint32_t o = G >> 1; o > 0; o >>= 1) {
-                bb.x = fminf(bb.x, __shfl_xor(bb.x, (int)o, 64));
-                bb.y = fminf(bb.y, __shfl_xor(bb.y, (int)o, 64));
-                bb.z = fmaxf(bb.z, __shfl_xor(bb.z, (int)o, 64));
-                bb.w = fmaxf(bb.w, __shfl_xor(bb.w, (int)o, 64));
-            }
-            uint8_t res = JG_Q_CLEAR;
-            bool done = !active;
-            if (active && box_relation(root_bb, bb) != REL_SURROUNDING) {
-                res = JG_Q_COLLISION;
-                done = true;
-            }
-            const int vr = active ? virtual_root(c, bb) : 0;
-            // edges of the pass's unfinished queries, descending from their virtual roots
-            // (cd_engine.rs:170-175); a query slot is marked hit as soon as one edge collides
-            wave_sync();
-            if (sub == 0) {
-                W.hit[slot] = done ? 1u : 0u;
-                if (FILT) W.qidx[slot] = (uint32_t)q;
-            }
-            // round R: lane (slot, sub) sets up edges sub + G (2R + j), j < WL_ENT / 64
-            constexpr uint32_t EPL = WL_ENT / WAVE;           // entities per lane per round
-            for (uint32_t i0 = 0; i0 < nv; i0 += EPL * G) {
-                wave_sync();
-                if (__ballot(active && W.hit[slot] == 0u) == 0ull) break;
-                uint32_t n = 0;
-                for (uint32_t j = 0; j < EPL; j++) {
-                    const uint32_t i = i0 + j * G + sub;
-                    const uint32_t o = j * WAVE + lane;
-                    const bool want = active && i < nv && W.hit[slot] == 0u;
-                    if (want) {
-                        const uint32_t k = i + 1 == nv ? 0 : i + 1;
-                        const float2 a = xform(m, it.v[i].x, it.v[i].y);
-                        const float2 b2 = xform(m, it.v[k].x, it.v[k].y);
-                        W.geom[o] = make_float4(a.x, a.y, b2.x, b2.y);
-                        W.ent_slot[o] = (uint8_t)slot;
-                    }
-                    n = wave_push_roots(W, n, want, o, (uint32_t)vr, lane);
-                }
-                wave_traverse<FILT, KIND_EDGE>(c, f, W, n, lane);
-            }
-            // containment over the virtual root's hazards (cd_engine.rs:176-193)
-            wave_sync();
-            done = W.hit[slot] != 0u || done;
-            if (active && W.hit[slot] != 0u && res == JG_Q_CLEAR) res = JG_Q_COLLISION;
-            bool hit = false;
-            if (!done) {
-                const DevNode vn = c.nodes[vr];
-                const uint32_t cntv = vn.haz_info & COUNT_MASK;
-                if (cntv > 0 && (vn.haz_info & ENTIRE_BIT)) {
-                    res = JG_Q_PANIC_FILTERED_ENTIRE;
-                    done = true;
-                } else {
-                    const float2 qpoi = xform(m, it.poi.x, it.poi.y);
-                    for (uint32_t i = sub; i < cntv; i += G) {
-                        const uint32_t slot_h = c.ent[vn.haz_begin + i].slot_flags & SLOT_MASK;
-                        if (irrel<FILT>(f, q, slot_h)) continue;
-                        if (containment(c, slot_h, it, m, bb, qpoi)) { hit = true; break; }
-                    }
-                }
-            }
-            const uint64_t bal = ballot(hit);
-            if (!done) res = ((bal >> (slot * G)) & gmask) ? JG_Q_COLLISION : JG_Q_CLEAR;
-            if (active && sub == 0) status[sd.first + q] = res;
-        }
+        poly_queries<FILT>(sets[gr.set], gr.q0, gr.mask, W, m6, status, lane);
+    }
+}
+
+// Both stages in one persistent kernel: a wave runs a work unit's surrogate stage and then the
+// polygon stage of that unit's survivors itself, so neither stage waits for the other's tail
+// and no survivor queue is written.
+// Items per lane of the fused kernel's surrogate stage: with 2 the kernel needs 129-130 VGPRs,
+// just over the 128 that keep 4 waves per SIMD.
+constexpr int FUSED_IPL = 1;
+template <bool FILT, int KIND>
+__global__ __launch_bounds__(WAVE) void k_fused(const SetDesc* __restrict__ sets, const Unit* __restrict__ units,
+                                               uint32_t n_units, uint32_t* __restrict__ counters,
+                                               const float* __restrict__ m6, uint8_t* __restrict__ status) {
+    const uint32_t lane = threadIdx.x;
+    __shared__ WaveLds lds;
+    const WorkList W = wave_list(lds);
+    for (;;) {
+        uint32_t u = 0;
+        if (lane == 0) u = atomicAdd(&counters[C_UNITS], 1u);
+        u = wave_bcast(u);
+        if (u >= n_units) return;
+        const Unit un = units[u];
+        const SetDesc& sd = sets[un.set];
+        const uint64_t todo = surrogate_unit<FILT, KIND, FUSED_IPL>(sd, un, W, m6, status, lane);
+        if (todo != 0ull) poly_queries<FILT>(sd, un.q0, todo, W, m6, status, lane);
+        wave_sync();
     }
 }
 
@@ -430,6 +475,20 @@ jg_status plan_launch(jg_plan* p, const float* d_m6, uint8_t* d_status, hipStrea
     const SetDesc* sets = (const SetDesc*)base;
     const Unit* units = (const Unit*)(base + p->off_units);
     const dim3 gs(p->grid_s), gp(p->grid_p), blk(WAVE);
+    if (p->cfg.stages != 2u) {
+#define JG_FUSED(F, K) hipLaunchKernelGGL((k_fused<F, K>), gs, blk, 0, stream, sets, units, p->n_units, counters, d_m6, d_status)
+        if (p->filtered) {
+            if (p->kind == KIND_CIRCLE) JG_FUSED(true, KIND_CIRCLE);
+            else if (p->kind == KIND_EDGE) JG_FUSED(true, KIND_EDGE);
+            else JG_FUSED(true, KIND_MIXED);
+        } else {
+            if (p->kind == KIND_CIRCLE) JG_FUSED(false, KIND_CIRCLE);
+            else if (p->kind == KIND_EDGE) JG_FUSED(false, KIND_EDGE);
+            else JG_FUSED(false, KIND_MIXED);
+        }
+#undef JG_FUSED
+        return hip_check(hipGetLastError(), "batched query launch");
+    }
 #define JG_SURR(F, K) hipLaunchKernelGGL((k_surrogate<F, K>), gs, blk, 0, stream, sets, units, p->n_units, counters, \
                                          groups, d_m6, d_status)
     if (p->filtered) {
@@ -597,16 +656,19 @@ extern "C" jg_status jg_plan_autotune(jg_plan* p, const float* d_m6, uint8_t* d_
     const std::vector<uint32_t> lanes = {0u, 8u, 16u, 32u, 64u};
     const std::vector<uint32_t> wgs = {4u * cus, 8u * cus, 16u * cus, 32u * cus};
     const std::vector<uint32_t> chunks = {16u, 32u, 64u};
-    // the whole grid (60 schedules): timings are noisy enough that coordinate descent can stop
-    // in a poor corner
+    // the whole grid (20 fused and 60 two-kernel schedules): timings are noisy enough that
+    // coordinate descent can stop in a poor corner
+    for (uint32_t sv : {1u, 2u})
     for (uint32_t lv : lanes)
         for (uint32_t wv : wgs)
             for (uint32_t cv : chunks) {
                 if (st != JG_OK) break;
+                if (sv == 1u && cv != chunks.front()) continue;       // one kernel: no survivor groups
                 jg_plan_config c = p->cfg;
+                c.stages = sv;
                 c.lanes_per_query = lv;
                 c.workgroups = wv;
-                c.chunk = cv;
+                c.chunk = sv == 1u ? p->cfg.chunk : cv;
                 p->cfg = c;
                 float ms = 0.0f;
                 st = plan_time(p, d_m6, d_status, reps, s, a, b, ms);

@@ -408,15 +408,25 @@ class CDEngine:
                                            _ptr(st, C.c_uint8)))
         return st
 
-    def collect_batch(self, item, mats, capacity=16, stream=None):
+    def collect_batch(self, item, mats, capacity=16, stream=None, out=None):
         """Per transform i: (set size, set hash, first `capacity` slot keys ascending) of
-        collect_surrogate_collisions ∪ collect_poly_collisions (ci_bench.rs:56-69)."""
+        collect_surrogate_collisions ∪ collect_poly_collisions (ci_bench.rs:56-69).  With
+        device transforms, `out` may give the (count int32 [n], hash int64 [n], keys int32
+        [n, capacity]) tensors to write into (contiguous rows)."""
         if _is_torch(mats):
             import torch
             n = mats.shape[0]
-            cnt = torch.empty(n, dtype=torch.int32, device=mats.device)
-            hsh = torch.empty(n, dtype=torch.int64, device=mats.device)
-            keys = torch.empty((n, max(1, capacity)), dtype=torch.int32, device=mats.device)
+            if out is not None:
+                cnt, hsh, keys = out
+                if cnt.shape != (n,) or hsh.shape != (n,) or keys.shape != (n, max(1, capacity)) or \
+                        (cnt.dtype, hsh.dtype, keys.dtype) != (torch.int32, torch.int64, torch.int32) or \
+                        any(t.device != mats.device or not t.is_contiguous() for t in (cnt, hsh, keys)):
+                    raise ValueError("out must be contiguous int32 [n], int64 [n], int32 [n, capacity] on the "
+                                     "transforms' device")
+            else:
+                cnt = torch.empty(n, dtype=torch.int32, device=mats.device)
+                hsh = torch.empty(n, dtype=torch.int64, device=mats.device)
+                keys = torch.empty((n, max(1, capacity)), dtype=torch.int32, device=mats.device)
             check(N.lib().jg_collect_batch(self._h, item._h, C.c_void_p(mats.data_ptr()), n, capacity,
                                            C.c_void_p(cnt.data_ptr()), C.c_void_p(hsh.data_ptr()),
                                            C.c_void_p(keys.data_ptr()) if capacity else None, _stream_ptr(stream)))
@@ -434,7 +444,7 @@ class CDEngine:
 class QueryPlan:
     """Batched persistent form: many (engine, item, transform range) sets in one launch."""
 
-    def __init__(self, sets, chunk=0, workgroups=0, lanes_per_query=0, flags=0):
+    def __init__(self, sets, chunk=0, workgroups=0, lanes_per_query=0, flags=0, stages=0):
         n = len(sets)
         arr = (N.QuerySetC * n)()
         self._keep = []
@@ -446,7 +456,7 @@ class QueryPlan:
             f = (filt or NoFilter())
             self._keep.append(f)
             arr[i].filter = f._filter()
-        cfg = N.PlanConfigC(chunk, workgroups, lanes_per_query, flags)
+        cfg = N.PlanConfigC(chunk, workgroups, lanes_per_query, flags, stages)
         self._h = C.c_void_p()
         self._refs = [s[0] for s in sets] + [s[1] for s in sets]
         check(N.lib().jg_plan_create(arr, n, C.byref(cfg), C.byref(self._h)))
@@ -483,7 +493,7 @@ class QueryPlan:
     @staticmethod
     def _cfg_dict(c):
         return {"chunk": c.chunk, "workgroups": c.workgroups, "lanes_per_query": c.lanes_per_query,
-                "flags": c.flags}
+                "flags": c.flags, "stages": c.stages}
 
 
 class PlacementSearch:

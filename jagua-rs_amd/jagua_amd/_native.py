@@ -58,7 +58,7 @@ class QuerySetC(C.Structure):
 
 class PlanConfigC(C.Structure):
     _fields_ = [("chunk", C.c_uint32), ("workgroups", C.c_uint32), ("lanes_per_query", C.c_uint32),
-                ("flags", C.c_uint32)]
+                ("flags", C.c_uint32), ("stages", C.c_uint32)]
 
 
 class SearchConfigC(C.Structure):
