@@ -302,6 +302,7 @@ extern "C" jg_status jg_search_create(const jg_search_item* items, uint32_t n_it
         sets[i] = jg_query_set{it.cde, it.item, (uint64_t)i * cfg->samples_per_round, cfg->samples_per_round, it.filter};
     }
     jg_plan_config pc{};
+    pc.stages = 2;          // rounds are small launches: the survivor queue spreads the polygon stage
     jg_status st = jg_plan_create(sets.data(), n_items, &pc, &S->plan);
     if (st != JG_OK) return st;
     *out = S.release();
