@@ -708,7 +708,7 @@ extern "C" jg_status jg_cde_dump(const jg_cde* c, uint32_t* nodes, uint64_t node
     *n_haz = f.ent.size();
     for (size_t i = 0; nodes && i < f.nodes.size() && i < nodes_cap; i++) {
         uint32_t* o = nodes + 8 * i;
-        // level is recovered from the BFS structure: root level = depth, children one less
+        // level is recovered from the breadth-first structure: root level = depth, children one less
         o[0] = 0;
         o[1] = f.nodes[i].first_child >= 0 ? 4 : 0;
         o[2] = f.nodes[i].haz_info & 0x3fffffffu;

@@ -81,7 +81,7 @@ __device__ __forceinline__ int strongest(const CdeView& c, const DevNode& nd, co
 
 // Depth-first descent in the reference's quadrant order without a memory stack: the
 // pending-sibling masks of the current path live 4 bits per level in one 64-bit register;
-// moving to a sibling or the parent uses the node_up links of the BFS image.
+// moving to a sibling or the parent uses the node_up links of the device image.
 // visit(node, &first_child) returns HIT to stop, or the mask of children to descend into.
 template <class Visit>
 __device__ __forceinline__ bool descend(const CdeView& c, int start, Visit visit) {

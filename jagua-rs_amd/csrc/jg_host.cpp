@@ -1,5 +1,6 @@
 // Host build of the collision engine: SPolygon construction with its surrogate, the hazard
-// slot map, the quadtree node pool under register/deregister, and the BFS flattener.
+// slot map, the quadtree node pool under register/deregister, the breadth-first flattener
+// (tree dumps) and the pool-ordered device image with its in-place patches.
 // Arithmetic follows the reference's f32 operation order (compiled -ffp-contract=off, no
 // fast-math, sinf/cosf as run-time libm calls).
 #include "jg_host.hpp"
@@ -591,7 +592,7 @@ void QuadTree::deregister_hazard(int32_t node, uint32_t slot, uint32_t version) 
 }
 
 // ---------------------------------------------------------------------------------------
-// BFS flattening
+// breadth-first flattening (the tree dump's order)
 // ---------------------------------------------------------------------------------------
 void flatten(const QuadTree& qt, const SlotMap& map, FlatCDE& f) {
     f = FlatCDE{};

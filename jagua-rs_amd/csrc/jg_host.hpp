@@ -1,6 +1,7 @@
 // Host-side data structures of the MI355X collision engine: polygons (SPolygon + surrogate),
 // the hazard slot map, the mutable quadtree (a node pool that keeps the reference's exact
-// structure under register / deregister) and its flattened BFS structure-of-arrays image.
+// structure under register / deregister) and its device image: pool-ordered records with
+// in-place updates, and the breadth-first flattening that the tree dump uses.
 #pragma once
 #include <stdint.h>
 
@@ -110,7 +111,7 @@ struct QuadTree {
     void push_to_children(int32_t node, const QEntry& e, const SlotMap& map);
 };
 
-// Flattened BFS image uploaded to HBM.
+// Records of the device image (and of the breadth-first flattening).
 struct FlatEntry {                       // 32 B
     float4 ff;
     uint32_t slot_flags;                 // slot | 0x80000000 when Entire
@@ -122,7 +123,7 @@ struct FlatEntry {                       // 32 B
 // One quadtree node, 32 B: everything a descent step reads in one load.
 struct FlatNode {
     float4 bbox;
-    int32_t first_child;                 // BFS index of the first of 4 children, -1 for a leaf
+    int32_t first_child;                 // index of the first of 4 consecutive children, -1 for a leaf
     uint32_t haz_begin;                  // first hazard entry
     uint32_t haz_info;                   // entry count | 0x80000000 when the first entry is Entire
     uint32_t n_active_edges;             // partial edges over all entries (saturated)
