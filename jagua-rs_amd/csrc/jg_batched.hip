@@ -73,8 +73,17 @@ __device__ __forceinline__ WorkList wave_list(WaveLds& L) {
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
+// Lane 0's f32 value, bit for bit, in a scalar register (readfirstlane moves 32-bit integers).
+__device__ __forceinline__ float uniform_f32(float x) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+
 template <typename T>
-__device__ __forceinline__ T wave_bcast(T v) { return (T)__shfl((int)v, 0, 64); }
+__device__ __forceinline__ T wave_bcast(T v) {
+    // lane 0's value in a scalar register: what is derived from it (work-unit records, the set
+    // descriptor and the engine's pointers) stays wave-uniform and is read with scalar loads
+    return (T)__builtin_amdgcn_readfirstlane((int)v);
+}
 
 }  // namespace
 
@@ -217,7 +226,15 @@ __device__ __forceinline__ void poly_queries(const SetDesc& sd, uint32_t q0, uin
             res = JG_Q_COLLISION;
             done = true;
         }
-        const int vr = active ? virtual_root(c, bb) : 0;
+        int vr = 0;
+        if (G == 64u) {
+            // one query per wavefront: its bbox is wave-uniform, so the descent's node reads go
+            // through the scalar cache instead of a vector load per level
+            const float4 ub = make_float4(uniform_f32(bb.x), uniform_f32(bb.y), uniform_f32(bb.z), uniform_f32(bb.w));
+            if (__builtin_amdgcn_readfirstlane(active ? 1 : 0)) vr = virtual_root(c, ub);
+        } else if (active) {
+            vr = virtual_root(c, bb);
+        }
         // edges of the pass's unfinished queries, descending from their virtual roots
         // (cd_engine.rs:170-175); a query slot is marked hit as soon as one edge collides
         wave_sync();

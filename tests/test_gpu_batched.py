@@ -123,3 +123,29 @@ def test_autotune_keeps_verdicts(cuda):
     plan.detect_graph(m, out, replays=2)
     torch.cuda.synchronize()
     np.testing.assert_array_equal(out.cpu().numpy(), np.concatenate(expect))
+
+
+def test_batched_fractional_small_layout(cuda):
+    """A layout tens of units wide (leaf nodes about 1.25 units) with 48-64-vertex query
+    items, so each query fills a wavefront in the polygon stage: the query bbox and virtual
+    root must be computed from exact f32 values (a bbox rounded to integers picks a wrong
+    virtual root here), plain and batched forms against the oracle."""
+    sc = S.generate_scene(depth=5, seed=41, container_kind=0, cw=40.0, ch=16.0, n_items=40, item_nv_lo=12,
+                          item_nv_hi=40, item_r_lo=0.8, item_r_hi=1.6, item_shape=0, min_sep=0.0)
+    cde, keys, oc = engines(sc)
+    polys = [S.polygon(42 + i, 48, 64, 0.6, 1.4, i % 3) for i in range(4)]
+    prod, orc = items(polys)
+    sets, refs, mats_all, first = [], [], [], 0
+    for i, (it, oit) in enumerate(zip(prod, orc)):
+        m = J.compose(S.transforms(43 + i, 4096, (0, 0, 40, 16), 0))
+        refs.append(O.query(oc, oit, mats=m, mode=1, threads=8)["status"])
+        sets.append((cde, it, first, len(m), None))
+        mats_all.append(m)
+        first += len(m)
+    ref = np.concatenate(refs)
+    assert 0 < ref.sum() < len(ref)
+    M = torch.from_numpy(np.concatenate(mats_all)).to(cuda)
+    for stages in (1, 2):
+        out = torch.full((len(M),), 9, dtype=torch.uint8, device=cuda)
+        J.QueryPlan(sets, stages=stages).detect(M, out)
+        np.testing.assert_array_equal(out.cpu().numpy(), ref, err_msg=f"stages {stages}")
