@@ -9,8 +9,9 @@ Outputs:
     oracle/_build/liboracle.so          CPU oracle, built by oracle/Makefile
 
 Device code is compiled for gfx950 only, with -ffp-contract=off (no FMA contraction: the
-reference never fuses), IEEE f32 division/sqrt and denormals kept (the HIP defaults), and
-packed-FP32 instruction selection disabled.
+reference never fuses), IEEE f32 division/sqrt (the HIP default), f32 denormals kept
+(-fno-gpu-flush-denormals-to-zero, also the default, stated so that no toolchain default can
+change it), and packed-FP32 instruction selection disabled.
 """
 import concurrent.futures as cf
 import os
@@ -34,7 +35,7 @@ COMMON = [
 ]
 DEVICE = [
     f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-munsafe-fp-atomics",
+    "-fno-gpu-flush-denormals-to-zero", "-munsafe-fp-atomics",
 ]
 
 SOURCES = ["jg_host.cpp", "jg_api.hip", "jg_batched.hip", "jg_overlap.hip", "jg_poles.hip", "jg_search.hip", "jg_piers.hip"]
