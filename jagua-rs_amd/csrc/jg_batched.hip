@@ -326,8 +326,8 @@ __global__ __launch_bounds__(WAVE, 4) void k_poly_filtered(const SetDesc* __rest
 // Both stages in one persistent kernel: a wave runs a work unit's surrogate stage and then the
 // polygon stage of that unit's survivors itself, so neither stage waits for the other's tail
 // and no survivor queue is written.
-// Items per lane of the fused kernel's surrogate stage: with 2 the kernel needs 129-130 VGPRs,
-// just over the 128 that keep 4 waves per SIMD.
+// Items per lane of the fused kernel's surrogate stage.  2 fits in 90 VGPRs (4 waves per SIMD
+// kept) but measured 2% slower on the configuration-2 step than 1.
 constexpr int FUSED_IPL = 1;
 template <bool FILT, int KIND>
 __device__ __forceinline__ void fused_loop(const SetDesc* __restrict__ sets, const Unit* __restrict__ units,
