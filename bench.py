@@ -203,7 +203,9 @@ def run_c2(args):
         achieved = tot_bytes / (launch_ms / 1e3) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic("pmc_batched_config2.md"),
-                "kernel": "k_fused<KIND_CIRCLE> (one batched step)", "launch_ms": launch_ms,
+                "kernel": ("k_fused<KIND_CIRCLE> (one batched step)" if tune["schedule"]["stages"] == 1 else
+                           "k_surrogate<false, KIND_CIRCLE> + k_poly (one batched step, two launches)"),
+                "launch_ms": launch_ms,
                 "algorithmic_bytes_per_launch": tot_bytes}
         cpu = {"value": total / cpu_sec, "unit": "queries/s", "cores": threads, "kind": "port",
                "sample": f"the full step ({total} detect queries) on the C++ oracle, {threads} threads, best of 2",

@@ -252,12 +252,14 @@ typedef struct {
 } jg_query_set;
 
 typedef struct {
-    uint32_t chunk;                       /* queries per polygon-stage group: 16, 32, 64 (0 = 32) */
+    uint32_t chunk;                       /* queries per work unit: 16, 32, 64 (0 = 64) */
     uint32_t workgroups;                  /* persistent one-wavefront workgroups (0 = 16 per compute unit) */
-    uint32_t lanes_per_query;             /* 0 = per item, from its vertex count */
+    uint32_t lanes_per_query;             /* 0 = per item: its vertex count rounded up to a power
+                                             of two, 8..32 */
     uint32_t flags;                       /* JG_QUERY_* */
-    uint32_t stages;                      /* 0 or 1: one kernel runs both stages of a work unit;
-                                             2: surrogate and polygon stages as two kernels */
+    uint32_t stages;                      /* 0 or 2: surrogate and polygon stages as two kernels,
+                                             survivors passed in groups of one polygon pass;
+                                             1: one kernel runs both stages of a work unit */
 } jg_plan_config;
 
 jg_status jg_plan_create(const jg_query_set* sets, uint32_t n_sets, const jg_plan_config* cfg, jg_plan** out);
@@ -269,10 +271,10 @@ jg_status jg_plan_detect_graph(jg_plan* plan, const float* d_mat6, uint8_t* d_st
                                void* stream);
 /* Schedule autotuner (no reference counterpart: the reference evaluates one query at a
  * time).  Times the plan's launch on `stream` with HIP events for candidate schedules —
- * stage layout {one kernel, two kernels}, lanes per query in the polygon stage {per item, 8,
- * 16, 32, 64}, workgroups {4, 8, 16, 32} per compute unit and, with two kernels, queries per
- * survivor group {16, 32, 64}: all 80 combinations plus the current one (median of `reps` timed launches each, after one warm-up), and keeps the
- * fastest in the plan.  The plan's flags are kept.  Every schedule computes the same verdicts;
+ * stage layout {two kernels, one kernel}, lanes per query in the polygon stage {per item, 8,
+ * 16, 32, 64}, workgroups {4, 8, 16, 32} per compute unit and queries per work unit {16, 32,
+ * 64}: all 120 combinations plus the current one (median of `reps` timed launches each, after
+ * one warm-up), and keeps the fastest in the plan.  The plan's flags are kept.  Every schedule computes the same verdicts;
  * d_status holds them afterwards.  `best` and `best_ms` may be NULL. */
 jg_status jg_plan_autotune(jg_plan* plan, const float* d_mat6, uint8_t* d_status, uint32_t reps, void* stream,
                            jg_plan_config* best, float* best_ms);

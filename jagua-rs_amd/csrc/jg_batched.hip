@@ -2,17 +2,17 @@
 // in one launch sequence fed by device work queues.
 //   k_surrogate: work units of up to 64 queries; lanes = (query, ff pole/pier) pairs, each
 //                lane descends from the root (cd_engine.rs:204-223).  Collisions are written
-//                out; the survivors of a unit are appended to the survivor queue as
-//                (set, first query, mask) groups of up to `chunk` queries each.
+//                out; the survivors of a unit are appended to the survivor queue in groups of
+//                one polygon-stage pass each (64/G queries, (set, first query, mask)).
 //   k_poly:      survivor groups; each query is handled by G lanes (G = the item's vertex
-//                count rounded up to a power of two, 8..64), 64/G queries per wavefront:
-//                transformed bbox, virtual root, the item's edges descending from it with a
-//                group-wide early exit, then containment over the virtual root's hazards
+//                count rounded up to a power of two, 8..32 by default), 64/G queries per
+//                wavefront: transformed bbox, virtual root, the item's edges descending from it
+//                with a group-wide early exit, then containment over the virtual root's hazards
 //                (cd_engine.rs:160-197).
-// k_fused runs both stages of a work unit in one wave (the default schedule); the two-kernel
-// form (k_surrogate, then k_poly over the survivor queue) is the alternative the autotuner
-// weighs.  Units are ordered by estimated
-// cost (largest items first) so the tail stays short.
+// The default schedule is these two kernels.  k_fused runs both stages of a work unit in one
+// wave instead: no survivor queue, but a unit with many survivors keeps its wave busy long
+// after the others finish; the autotuner weighs it.  Units are ordered by item size (largest
+// first).  Work is claimed from 8 interleaved streams of counters, never from one address.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -43,7 +43,7 @@ struct Unit {              // 16 B
     uint32_t set;
     uint32_t q0;           // first query, relative to the set
     uint32_t count;        // <= 64
-    uint32_t group;        // queries per survivor group (16, 32 or 64)
+    uint32_t pad;
 };
 
 struct Group {             // 16 B: surviving queries q0 + i for the set bits i of mask
@@ -52,8 +52,19 @@ struct Group {             // 16 B: surviving queries q0 + i for the set bits i 
     uint64_t mask;
 };
 
-// counters: [0] surrogate units taken, [1] groups appended, [2] groups taken
-constexpr int C_UNITS = 0, C_GROUPS = 1, C_TAKEN = 2;
+// Work is claimed from 8 streams, each with its own counters on their own 128-byte lines:
+// thousands of waves taking work with atomics on a single address queue behind each other
+// (measured: 1.81 ms against 1.30 ms per configuration-2 step for the survivor groups alone).
+//   unit stream x:  units x + 8 j; counter C_USTREAM0 + 32 x counts the units taken
+//   group stream x: the survivor groups of the units of unit stream x, in their own region of
+//                   the group array (base gbase[x]); C_GCOUNT0 + 32 x counts the groups
+//                   appended, C_STREAM0 + 32 x the groups taken
+// A wave starts on stream blockIdx.x mod 8 and moves to the next one as each runs dry.
+constexpr uint32_t N_STREAMS = 8, STREAM_STRIDE = 32;
+constexpr uint32_t C_STREAM0 = 0;
+constexpr uint32_t C_USTREAM0 = C_STREAM0 + STREAM_STRIDE * N_STREAMS;
+constexpr uint32_t C_GCOUNT0 = C_USTREAM0 + STREAM_STRIDE * N_STREAMS;
+constexpr uint32_t N_COUNTER_WORDS = C_GCOUNT0 + STREAM_STRIDE * N_STREAMS;
 
 // LDS of a workgroup (one wavefront): its work list
 constexpr uint32_t WAVE = 64;
@@ -87,8 +98,24 @@ __device__ __forceinline__ T wave_bcast(T v) {
 
 }  // namespace
 
+// The next item of n from 8 interleaved streams (item x + 8 j in stream x, counters from word
+// base on): the wave's stream x first, then the following ones as they drain; n when all are.
+__device__ __forceinline__ uint32_t claim_next(uint32_t* counters, uint32_t base, uint32_t& x, uint32_t& tried,
+                                               uint32_t n, uint32_t lane) {
+    while (tried < N_STREAMS) {
+        uint32_t j = 0;
+        if (lane == 0) j = atomicAdd(&counters[base + STREAM_STRIDE * x], 1u);
+        j = __builtin_amdgcn_readfirstlane(j);
+        const uint64_t i = (uint64_t)x + (uint64_t)N_STREAMS * j;
+        if (i < n) return (uint32_t)i;
+        x = (x + 1u) & (N_STREAMS - 1u);
+        tried++;
+    }
+    return n;
+}
+
 __global__ void k_reset_counters(uint32_t* counters) {
-    if (threadIdx.x < 4) counters[threadIdx.x] = 0u;
+    for (uint32_t i = threadIdx.x; i < N_COUNTER_WORDS; i += blockDim.x) counters[i] = 0u;
 }
 
 // The surrogate stage of one work unit (cd_engine.rs:204-223 for each of its queries): writes
@@ -158,25 +185,32 @@ __device__ __forceinline__ uint64_t surrogate_unit(const SetDesc& sd, const Unit
 template <bool FILT, int KIND>
 __global__ __launch_bounds__(WAVE) void k_surrogate(const SetDesc* __restrict__ sets, const Unit* __restrict__ units,
                                                    uint32_t n_units, uint32_t* __restrict__ counters,
-                                                   Group* __restrict__ groups, const float* __restrict__ m6,
-                                                   uint8_t* __restrict__ status) {
+                                                   Group* __restrict__ groups, const uint32_t* __restrict__ gbase,
+                                                   const float* __restrict__ m6, uint8_t* __restrict__ status) {
     const uint32_t lane = threadIdx.x;
     __shared__ WaveLds lds;
     const WorkList W = wave_list(lds);
+    uint32_t ux = blockIdx.x & (N_STREAMS - 1u), utried = 0;
     for (;;) {
-        uint32_t u = 0;
-        if (lane == 0) u = atomicAdd(&counters[C_UNITS], 1u);
-        u = wave_bcast(u);
+        const uint32_t u = claim_next(counters, C_USTREAM0, ux, utried, n_units, lane);
         if (u >= n_units) return;
         const Unit un = units[u];
         const uint64_t todo = surrogate_unit<FILT, KIND, 2>(sets[un.set], un, W, m6, status, lane);
-        if (todo != 0ull && lane == 0)
-            for (uint32_t b = 0; b < un.count; b += un.group) {
-                const uint64_t sub = (todo >> b) & (un.group >= 64u ? ~0ull : ((1ull << un.group) - 1ull));
-                if (sub == 0ull) continue;
-                const uint32_t gi = atomicAdd(&counters[C_GROUPS], 1u);
-                groups[gi] = Group{un.set, un.q0 + b, sub};
+        if (todo != 0ull && lane == 0) {
+            // one survivor group per polygon-stage pass (64 / G queries)
+            const uint32_t per = 64u / sets[un.set].lanes;
+            const uint32_t n_g = ((uint32_t)__builtin_popcountll(todo) + per - 1u) / per;
+            // the unit's stream (u mod 8) appends to its own region of the group array
+            const uint32_t xs = u & (N_STREAMS - 1u);
+            const uint32_t g0 = gbase[xs] + atomicAdd(&counters[C_GCOUNT0 + STREAM_STRIDE * xs], n_g);
+            uint64_t rest = todo;
+            for (uint32_t j = 0; j < n_g; j++) {
+                uint64_t r = rest;
+                for (uint32_t i = 0; i < per && r; i++) r &= r - 1ull;
+                groups[g0 + j] = Group{un.set, un.q0, rest & ~r};
+                rest = r;
             }
+        }
         wave_sync();
     }
 }
@@ -291,17 +325,23 @@ __device__ __forceinline__ void poly_queries(const SetDesc& sd, uint32_t q0, uin
 
 template <bool FILT>
 __device__ __forceinline__ void poly_loop(const SetDesc* __restrict__ sets, uint32_t* __restrict__ counters,
-                                          const Group* __restrict__ groups, const float* __restrict__ m6,
-                                          uint8_t* __restrict__ status, WaveLds& lds) {
+                                          const Group* __restrict__ groups, const uint32_t* __restrict__ gbase,
+                                          const float* __restrict__ m6, uint8_t* __restrict__ status, WaveLds& lds) {
     const uint32_t lane = threadIdx.x;
     const WorkList W = wave_list(lds);
-    const uint32_t n_groups = __atomic_load_n(&counters[C_GROUPS], __ATOMIC_RELAXED);
-    for (;;) {
-        uint32_t gi = 0;
-        if (lane == 0) gi = atomicAdd(&counters[C_TAKEN], 1u);
-        gi = wave_bcast(gi);
-        if (gi >= n_groups) return;
-        const Group gr = groups[gi];
+    // stream x holds the groups its units appended; a wave drains its own stream, then the next
+    uint32_t x = blockIdx.x & (N_STREAMS - 1u);
+    for (uint32_t tried = 0; tried < N_STREAMS;) {
+        const uint32_t n_x = __atomic_load_n(&counters[C_GCOUNT0 + STREAM_STRIDE * x], __ATOMIC_RELAXED);
+        uint32_t j = 0;
+        if (lane == 0) j = atomicAdd(&counters[C_STREAM0 + STREAM_STRIDE * x], 1u);
+        j = wave_bcast(j);
+        if (j >= n_x) {
+            x = (x + 1u) & (N_STREAMS - 1u);
+            tried++;
+            continue;
+        }
+        const Group gr = groups[gbase[x] + j];
         poly_queries<FILT>(sets[gr.set], gr.q0, gr.mask, W, m6, status, lane);
     }
 }
@@ -311,21 +351,23 @@ __device__ __forceinline__ void poly_loop(const SetDesc* __restrict__ sets, uint
 // lands at 129) are bounded to that occupancy at the cost of a few spills, which measured 16%
 // faster on the filtered many-layout workload than 3 waves without spills.
 __global__ __launch_bounds__(WAVE) void k_poly(const SetDesc* __restrict__ sets, uint32_t* __restrict__ counters,
-                                              const Group* __restrict__ groups, const float* __restrict__ m6,
-                                              uint8_t* __restrict__ status) {
+                                              const Group* __restrict__ groups, const uint32_t* __restrict__ gbase,
+                                              const float* __restrict__ m6, uint8_t* __restrict__ status) {
     __shared__ WaveLds lds;
-    poly_loop<false>(sets, counters, groups, m6, status, lds);
+    poly_loop<false>(sets, counters, groups, gbase, m6, status, lds);
 }
 __global__ __launch_bounds__(WAVE, 4) void k_poly_filtered(const SetDesc* __restrict__ sets, uint32_t* __restrict__ counters,
-                                                          const Group* __restrict__ groups, const float* __restrict__ m6,
+                                                          const Group* __restrict__ groups,
+                                                          const uint32_t* __restrict__ gbase, const float* __restrict__ m6,
                                                           uint8_t* __restrict__ status) {
     __shared__ WaveLds lds;
-    poly_loop<true>(sets, counters, groups, m6, status, lds);
+    poly_loop<true>(sets, counters, groups, gbase, m6, status, lds);
 }
 
 // Both stages in one persistent kernel: a wave runs a work unit's surrogate stage and then the
-// polygon stage of that unit's survivors itself, so neither stage waits for the other's tail
-// and no survivor queue is written.
+// polygon stage of that unit's survivors itself.  No survivor queue is written, but the work of
+// a unit cannot spread: a unit of a large item with 30 survivors keeps one wave busy for about
+// 1.2 ms of a 1.34 ms configuration-2 step, so the two-kernel schedule is the default.
 // Items per lane of the fused kernel's surrogate stage.  2 fits in 90 VGPRs (4 waves per SIMD
 // kept) but measured 2% slower on the configuration-2 step than 1.
 constexpr int FUSED_IPL = 1;
@@ -336,10 +378,9 @@ __device__ __forceinline__ void fused_loop(const SetDesc* __restrict__ sets, con
                                            WaveLds& lds) {
     const uint32_t lane = threadIdx.x;
     const WorkList W = wave_list(lds);
+    uint32_t ux = blockIdx.x & (N_STREAMS - 1u), utried = 0;
     for (;;) {
-        uint32_t u = 0;
-        if (lane == 0) u = atomicAdd(&counters[C_UNITS], 1u);
-        u = wave_bcast(u);
+        const uint32_t u = claim_next(counters, C_USTREAM0, ux, utried, n_units, lane);
         if (u >= n_units) return;
         const Unit un = units[u];
         const SetDesc& sd = sets[un.set];
@@ -374,7 +415,7 @@ struct jg_plan {
     DevBlob masks;                  // JG_FILTER_KEYS masks of every set
     std::vector<uint64_t> generations;
     uint32_t n_units = 0;
-    size_t off_units = 0, off_counter = 0, off_groups = 0;
+    size_t off_units = 0, off_counter = 0, off_groups = 0, off_gbase = 0;
     uint32_t grid_s = 0, grid_p = 0;
     uint32_t n_cus = 0;
     bool filtered = false;          // some set has a filter (kernels with the filter test)
@@ -443,8 +484,10 @@ jg_status plan_build_table(jg_plan* p) {
         sd[i].first = s.first;
         uint32_t lanes = p->cfg.lanes_per_query;
         if (lanes == 0) {
+            // the vertex count rounded up to a power of two, 8..32: items above 32 vertices run
+            // two queries per pass, which measured faster than one query on 64 lanes
             lanes = 8;
-            while (lanes < s.item->poly.n() && lanes < 64u) lanes <<= 1;
+            while (lanes < s.item->poly.n() && lanes < 32u) lanes <<= 1;
         }
         uint32_t pow2 = 1;
         while (pow2 < lanes && pow2 < 64u) pow2 <<= 1;
@@ -466,10 +509,10 @@ jg_status plan_build_table(jg_plan* p) {
     }
     p->filtered = any_filter;
     p->kind = any_pier ? (any_pole ? KIND_MIXED : KIND_EDGE) : KIND_CIRCLE;
-    // work units: sets in decreasing estimated cost, 64 queries each (the surrogate stage packs
-    // 128 pole entities per traversal); survivors leave in groups of `chunk` queries
-    const uint32_t chunk = p->cfg.chunk ? std::min<uint32_t>(p->cfg.chunk, 64u) : 32u;
-    const uint32_t group = chunk <= 16u ? 16u : chunk <= 32u ? 32u : 64u;
+    // work units: sets in decreasing item size, `chunk` queries each (16, 32 or 64; 64 packs
+    // 128 pole entities into one surrogate traversal; smaller units shorten the fused kernel's
+    // tail)
+    const uint32_t uq = p->cfg.chunk == 0 ? 64u : p->cfg.chunk <= 16u ? 16u : p->cfg.chunk <= 32u ? 32u : 64u;
     std::vector<size_t> order(p->sets.size());
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
@@ -478,18 +521,32 @@ jg_status plan_build_table(jg_plan* p) {
     std::vector<Unit> units;
     for (size_t oi : order) {
         const jg_query_set& s = p->sets[oi];
-        for (uint64_t q = 0; q < s.count; q += 64u)
-            units.push_back(Unit{(uint32_t)oi, (uint32_t)q, (uint32_t)std::min<uint64_t>(64u, s.count - q), group});
+        for (uint64_t q = 0; q < s.count; q += uq)
+            units.push_back(Unit{(uint32_t)oi, (uint32_t)q, (uint32_t)std::min<uint64_t>(uq, s.count - q), 0u});
     }
     p->n_units = (uint32_t)units.size();
     const size_t o_units = (sizeof(SetDesc) * sd.size() + 255) & ~(size_t)255;
     const size_t o_ctr = (o_units + sizeof(Unit) * units.size() + 255) & ~(size_t)255;
-    const size_t o_grp = o_ctr + 256;
-    const size_t max_groups = units.size() * (64u / group);
+    const size_t o_gbase = o_ctr + 4u * N_COUNTER_WORDS;
+    // one group per polygon-stage pass at most; unit u's groups go to the region of stream u mod 8
+    uint64_t stream_groups[N_STREAMS] = {};
+    for (size_t u = 0; u < units.size(); u++) {
+        const uint32_t per = 64u / sd[units[u].set].lanes;
+        stream_groups[u & (N_STREAMS - 1u)] += (units[u].count + per - 1u) / per;
+    }
+    uint32_t gbase[N_STREAMS];
+    size_t max_groups = 0;
+    for (uint32_t x = 0; x < N_STREAMS; x++) {
+        gbase[x] = (uint32_t)max_groups;
+        max_groups += stream_groups[x];
+    }
+    if (max_groups >= 0xffffffffull) return fail(JG_ERR_CAPACITY, "too many survivor groups in a batched plan");
+    const size_t o_grp = (o_gbase + sizeof(gbase) + 255) & ~(size_t)255;
     const size_t total = o_grp + sizeof(Group) * (max_groups + 1);
     std::vector<uint8_t> host(total, 0);
     std::memcpy(host.data(), sd.data(), sizeof(SetDesc) * sd.size());
     std::memcpy(host.data() + o_units, units.data(), sizeof(Unit) * units.size());
+    std::memcpy(host.data() + o_gbase, gbase, sizeof(gbase));
     jg_status st = p->table.reserve(total);
     if (st != JG_OK) return st;
     st = hip_check(hipMemcpy(p->table.ptr, host.data(), total, hipMemcpyHostToDevice), "hipMemcpy plan");
@@ -497,6 +554,7 @@ jg_status plan_build_table(jg_plan* p) {
     p->off_units = o_units;
     p->off_counter = o_ctr;
     p->off_groups = o_grp;
+    p->off_gbase = o_gbase;
     p->generations = gens;
     if (p->gexec) {                 // a captured graph refers to the old table
         (void)hipGraphExecDestroy(p->gexec);
@@ -519,11 +577,12 @@ jg_status plan_launch(jg_plan* p, const float* d_m6, uint8_t* d_status, hipStrea
     uint8_t* base = (uint8_t*)p->table.ptr;
     uint32_t* counters = (uint32_t*)(base + p->off_counter);
     Group* groups = (Group*)(base + p->off_groups);
+    const uint32_t* gbase = (const uint32_t*)(base + p->off_gbase);
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream, counters);
     const SetDesc* sets = (const SetDesc*)base;
     const Unit* units = (const Unit*)(base + p->off_units);
     const dim3 gs(p->grid_s), gp(p->grid_p), blk(WAVE);
-    if (p->cfg.stages != 2u) {
+    if (p->cfg.stages == 1u) {
 #define JG_FUSED(KER, K) hipLaunchKernelGGL((KER<K>), gs, blk, 0, stream, sets, units, p->n_units, counters, d_m6, d_status)
         if (p->filtered) {
             if (p->kind == KIND_CIRCLE) JG_FUSED(k_fused_filtered, KIND_CIRCLE);
@@ -538,17 +597,18 @@ jg_status plan_launch(jg_plan* p, const float* d_m6, uint8_t* d_status, hipStrea
         return hip_check(hipGetLastError(), "batched query launch");
     }
 #define JG_SURR(F, K) hipLaunchKernelGGL((k_surrogate<F, K>), gs, blk, 0, stream, sets, units, p->n_units, counters, \
-                                         groups, d_m6, d_status)
+                                         groups, gbase, d_m6, d_status)
     if (p->filtered) {
         if (p->kind == KIND_CIRCLE) JG_SURR(true, KIND_CIRCLE);
         else if (p->kind == KIND_EDGE) JG_SURR(true, KIND_EDGE);
         else JG_SURR(true, KIND_MIXED);
-        hipLaunchKernelGGL(k_poly_filtered, gp, blk, 0, stream, sets, counters, (const Group*)groups, d_m6, d_status);
+        hipLaunchKernelGGL(k_poly_filtered, gp, blk, 0, stream, sets, counters, (const Group*)groups, gbase, d_m6,
+                           d_status);
     } else {
         if (p->kind == KIND_CIRCLE) JG_SURR(false, KIND_CIRCLE);
         else if (p->kind == KIND_EDGE) JG_SURR(false, KIND_EDGE);
         else JG_SURR(false, KIND_MIXED);
-        hipLaunchKernelGGL(k_poly, gp, blk, 0, stream, sets, counters, (const Group*)groups, d_m6, d_status);
+        hipLaunchKernelGGL(k_poly, gp, blk, 0, stream, sets, counters, (const Group*)groups, gbase, d_m6, d_status);
     }
 #undef JG_SURR
     return hip_check(hipGetLastError(), "batched query launch");
@@ -704,19 +764,18 @@ extern "C" jg_status jg_plan_autotune(jg_plan* p, const float* d_m6, uint8_t* d_
     const std::vector<uint32_t> lanes = {0u, 8u, 16u, 32u, 64u};
     const std::vector<uint32_t> wgs = {4u * cus, 8u * cus, 16u * cus, 32u * cus};
     const std::vector<uint32_t> chunks = {16u, 32u, 64u};
-    // the whole grid (20 fused and 60 two-kernel schedules): timings are noisy enough that
+    // the whole grid (60 two-kernel and 60 fused schedules): timings are noisy enough that
     // coordinate descent can stop in a poor corner
-    for (uint32_t sv : {1u, 2u})
+    for (uint32_t sv : {2u, 1u})
     for (uint32_t lv : lanes)
         for (uint32_t wv : wgs)
             for (uint32_t cv : chunks) {
                 if (st != JG_OK) break;
-                if (sv == 1u && cv != chunks.front()) continue;       // one kernel: no survivor groups
                 jg_plan_config c = p->cfg;
                 c.stages = sv;
                 c.lanes_per_query = lv;
                 c.workgroups = wv;
-                c.chunk = sv == 1u ? p->cfg.chunk : cv;
+                c.chunk = cv;
                 p->cfg = c;
                 float ms = 0.0f;
                 st = plan_time(p, d_m6, d_status, reps, s, a, b, ms);

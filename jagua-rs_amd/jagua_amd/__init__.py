@@ -442,7 +442,13 @@ class CDEngine:
 
 
 class QueryPlan:
-    """Batched persistent form: many (engine, item, transform range) sets in one launch."""
+    """Batched persistent form: many (engine, item, transform range) sets in one launch.
+
+    Schedule (jg_plan_config): chunk = queries per work unit (16, 32, 64; 0 = 64); workgroups
+    = persistent one-wave workgroups (0 = 16 per compute unit); lanes_per_query = lanes per
+    query in the polygon stage (0 = per item, 8..32); stages = 0 or 2 for two kernels (the
+    surrogate stage, then the polygon stage over one-pass survivor groups), 1 for one kernel
+    running both stages of a unit."""
 
     def __init__(self, sets, chunk=0, workgroups=0, lanes_per_query=0, flags=0, stages=0):
         n = len(sets)

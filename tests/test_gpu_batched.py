@@ -61,14 +61,17 @@ def test_batched_many_layouts(cuda):
     np.testing.assert_array_equal(out[: len(exp0)].cpu().numpy(), exp0)
 
 
-@pytest.mark.parametrize("chunk", [1, 7, 64])
-def test_batched_chunks(cuda, chunk):
+@pytest.mark.parametrize("stages,chunk,n", [(2, 1, 3000), (2, 32, 3000), (1, 7, 3000), (1, 64, 3000),
+                                             (2, 64, 100), (1, 16, 100), (0, 0, 1)])
+def test_batched_chunks(cuda, stages, chunk, n):
+    """Both stage layouts, every unit size, 7 workgroups (not a multiple of the 8 claim
+    streams) and plans with fewer units than streams: every query is claimed exactly once."""
     sc = S.config1_scene(20)
     cde, keys, oc = engines(sc)
-    polys, dts = S.config1_queries(20, n=3000)
+    polys, dts = S.config1_queries(20, n=n)
     prod, orc = items(polys)
     mats = J.compose(dts)
-    plan = J.QueryPlan([(cde, prod[0], 0, len(mats), None)], chunk=chunk, workgroups=7)
+    plan = J.QueryPlan([(cde, prod[0], 0, len(mats), None)], chunk=chunk, workgroups=7, stages=stages)
     out = torch.empty(len(mats), dtype=torch.uint8, device=cuda)
     plan.detect(torch.from_numpy(mats).to(cuda), out)
     ref = O.query(oc, orc[0], mats=mats, mode=1)["status"]
@@ -114,7 +117,7 @@ def test_autotune_keeps_verdicts(cuda):
     cfg, ms = plan.autotune(m, out, reps=2)
     torch.cuda.synchronize()
     assert ms > 0 and cfg == plan.config()
-    assert cfg["lanes_per_query"] in (0, 8, 16, 32, 64) and 1 <= cfg["chunk"] <= 64 or cfg["chunk"] == 0
+    assert cfg["lanes_per_query"] in (0, 8, 16, 32, 64) and cfg["chunk"] in (16, 32, 64) and cfg["stages"] in (1, 2)
     np.testing.assert_array_equal(out.cpu().numpy(), np.concatenate(expect))
     out.fill_(255)
     plan.detect(m, out)
