@@ -310,10 +310,37 @@ __device__ __forceinline__ void poly_queries(const SetDesc& sd, uint32_t q0, uin
                 done = true;
             } else {
                 const float2 qpoi = xform(m, it.poi.x, it.poi.y);
-                for (uint32_t i = sub; i < cntv; i += G) {
-                    const uint32_t slot_h = c.ent[vn.haz_begin + i].slot_flags & SLOT_MASK;
-                    if (irrel<FILT>(f, q, slot_h)) continue;
-                    if (containment(c, slot_h, it, m, bb, qpoi)) { hit = true; break; }
+                // detect_containment_collision of each candidate (containment() inlined), four
+                // candidates per lane in flight: a shallow virtual root holds up to every hazard
+                // of the layout, and one lane's scan of them is a chain of dependent reads
+                constexpr uint32_t CU = 4;
+                for (uint32_t i0 = sub; i0 < cntv && !hit; i0 += CU * G) {
+                    uint32_t sl[CU];
+                    float4 hb[CU];
+                    float iw[CU];
+#pragma unroll
+                    for (uint32_t r = 0; r < CU; r++) {
+                        const uint32_t i = i0 + r * G;
+                        sl[r] = c.ent[vn.haz_begin + (i < cntv ? i : 0u)].slot_flags & SLOT_MASK;
+                    }
+#pragma unroll
+                    for (uint32_t r = 0; r < CU; r++) {
+                        hb[r] = c.haz_bbox[sl[r]];
+                        iw[r] = c.haz_poi[sl[r]].w;
+                    }
+#pragma unroll
+                    for (uint32_t r = 0; r < CU; r++) {
+                        if (hit || i0 + r * G >= cntv || irrel<FILT>(f, q, sl[r])) continue;
+                        const int rel = box_almost_relation(hb[r], bb);
+                        bool contained = false;
+                        if (rel == REL_SURROUNDING) {
+                            contained = pip_hazard(c, sl[r], qpoi.x, qpoi.y);
+                        } else if (rel == REL_ENCLOSED) {
+                            const float4 hp = c.haz_poi[sl[r]];
+                            contained = pip_query(it, m, bb, hp.x, hp.y);
+                        }
+                        if (iw[r] != 0.0f ? contained : !contained) hit = true;
+                    }
                 }
             }
         }
