@@ -215,6 +215,52 @@ __global__ __launch_bounds__(WAVE) void k_surrogate(const SetDesc* __restrict__ 
     }
 }
 
+// detect_containment_collision (cd_engine.rs:231-255; containment() inlined) over the
+// candidates i = start, start + stride, ... < cnt of the entry list at e0; true at the first
+// that collides.  Each lane keeps four candidates in flight (their entries, then their bboxes
+// and interior flags, read before any test): a shallow virtual root holds up to every hazard of
+// the layout, and one candidate at a time is a chain of dependent reads.
+template <bool FILT>
+__device__ __forceinline__ bool scan_containment(const CdeView& c, const FilterView& f, uint64_t q, uint32_t e0,
+                                                 uint32_t cnt, uint32_t start, uint32_t stride, const ItemView& it,
+                                                 const Mat& m, float4 bb, float2 qpoi) {
+    constexpr uint32_t CU = 4;
+    bool hit = false;
+    for (uint32_t i0 = start; i0 < cnt && !hit; i0 += CU * stride) {
+        uint32_t sl[CU];
+        float4 hb[CU];
+        float iw[CU];
+#pragma unroll
+        for (uint32_t r = 0; r < CU; r++) {
+            const uint32_t i = i0 + r * stride;
+            sl[r] = c.ent[e0 + (i < cnt ? i : 0u)].slot_flags & SLOT_MASK;
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < CU; r++) {
+            hb[r] = c.haz_bbox[sl[r]];
+            iw[r] = c.haz_poi[sl[r]].w;
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < CU; r++) {
+            if (hit || i0 + r * stride >= cnt || irrel<FILT>(f, q, sl[r])) continue;
+            const int rel = box_almost_relation(hb[r], bb);
+            bool contained = false;
+            if (rel == REL_SURROUNDING) {
+                contained = pip_hazard(c, sl[r], qpoi.x, qpoi.y);
+            } else if (rel == REL_ENCLOSED) {
+                const float4 hp = c.haz_poi[sl[r]];
+                contained = pip_query(it, m, bb, hp.x, hp.y);
+            }
+            if (iw[r] != 0.0f ? contained : !contained) hit = true;
+        }
+    }
+    return hit;
+}
+
+// A query whose virtual root holds more containment candidates than this has them scanned by
+// the whole wave after the pass's other work, not by its own G lanes.
+constexpr uint32_t WAVE_SCAN_MIN = 64;
+
 // The polygon stage (cd_engine.rs:160-197) of the queries q0 + i for the set bits i of mask,
 // all of one set: 64/G queries per pass, G lanes each.
 template <bool FILT>
@@ -301,48 +347,41 @@ __device__ __forceinline__ void poly_queries(const SetDesc& sd, uint32_t q0, uin
         wave_sync();
         done = W.hit[slot] != 0u || done;
         if (active && W.hit[slot] != 0u && res == JG_Q_CLEAR) res = JG_Q_COLLISION;
-        bool hit = false;
+        bool hit = false, heavy = false;
+        uint32_t e0 = 0, cntv = 0;
         if (!done) {
             const DevNode vn = c.nodes[vr];
-            const uint32_t cntv = vn.haz_info & COUNT_MASK;
+            cntv = vn.haz_info & COUNT_MASK;
+            e0 = vn.haz_begin;
             if (cntv > 0 && (vn.haz_info & ENTIRE_BIT)) {
                 res = JG_Q_PANIC_FILTERED_ENTIRE;
                 done = true;
+            } else if (G < 64u && cntv > WAVE_SCAN_MIN) {
+                heavy = true;
             } else {
                 const float2 qpoi = xform(m, it.poi.x, it.poi.y);
-                // detect_containment_collision of each candidate (containment() inlined), four
-                // candidates per lane in flight: a shallow virtual root holds up to every hazard
-                // of the layout, and one lane's scan of them is a chain of dependent reads
-                constexpr uint32_t CU = 4;
-                for (uint32_t i0 = sub; i0 < cntv && !hit; i0 += CU * G) {
-                    uint32_t sl[CU];
-                    float4 hb[CU];
-                    float iw[CU];
-#pragma unroll
-                    for (uint32_t r = 0; r < CU; r++) {
-                        const uint32_t i = i0 + r * G;
-                        sl[r] = c.ent[vn.haz_begin + (i < cntv ? i : 0u)].slot_flags & SLOT_MASK;
-                    }
-#pragma unroll
-                    for (uint32_t r = 0; r < CU; r++) {
-                        hb[r] = c.haz_bbox[sl[r]];
-                        iw[r] = c.haz_poi[sl[r]].w;
-                    }
-#pragma unroll
-                    for (uint32_t r = 0; r < CU; r++) {
-                        if (hit || i0 + r * G >= cntv || irrel<FILT>(f, q, sl[r])) continue;
-                        const int rel = box_almost_relation(hb[r], bb);
-                        bool contained = false;
-                        if (rel == REL_SURROUNDING) {
-                            contained = pip_hazard(c, sl[r], qpoi.x, qpoi.y);
-                        } else if (rel == REL_ENCLOSED) {
-                            const float4 hp = c.haz_poi[sl[r]];
-                            contained = pip_query(it, m, bb, hp.x, hp.y);
-                        }
-                        if (iw[r] != 0.0f ? contained : !contained) hit = true;
-                    }
-                }
+                hit = scan_containment<FILT>(c, f, q, e0, cntv, sub, G, it, m, bb, qpoi);
             }
+        }
+        // the long candidate lists, one query at a time over all 64 lanes; the query's
+        // transform, bbox, entry range and index come from its first lane bit for bit
+        for (uint64_t hv = ballot(heavy && sub == 0); hv; hv &= hv - 1ull) {
+            const int l0 = __builtin_ctzll(hv);
+            Mat mq;
+            mq.a = __shfl(m.a, l0, 64);
+            mq.b = __shfl(m.b, l0, 64);
+            mq.c = __shfl(m.c, l0, 64);
+            mq.d = __shfl(m.d, l0, 64);
+            mq.e = __shfl(m.e, l0, 64);
+            mq.f = __shfl(m.f, l0, 64);
+            const float4 bq = make_float4(__shfl(bb.x, l0, 64), __shfl(bb.y, l0, 64), __shfl(bb.z, l0, 64),
+                                          __shfl(bb.w, l0, 64));
+            const uint32_t e0q = (uint32_t)__shfl((int)e0, l0, 64), cq = (uint32_t)__shfl((int)cntv, l0, 64);
+            const uint64_t qq = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(q >> 32), l0, 64) << 32) |
+                                (uint32_t)__shfl((int)(uint32_t)q, l0, 64);
+            const float2 pq = xform(mq, it.poi.x, it.poi.y);
+            const bool any = ballot(scan_containment<FILT>(c, f, qq, e0q, cq, lane, 64u, it, mq, bq, pq)) != 0ull;
+            if ((int)(lane / G) == l0 / (int)G) hit = any;
         }
         const uint64_t bal = ballot(hit);
         if (!done) res = ((bal >> (slot * G)) & gmask) ? JG_Q_COLLISION : JG_Q_CLEAR;
