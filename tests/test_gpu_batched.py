@@ -152,3 +152,41 @@ def test_batched_fractional_small_layout(cuda):
         out = torch.full((len(M),), 9, dtype=torch.uint8, device=cuda)
         J.QueryPlan(sets, stages=stages).detect(M, out)
         np.testing.assert_array_equal(out.cpu().numpy(), ref, err_msg=f"stages {stages}")
+
+
+@pytest.mark.parametrize("filtered", [False, True])
+def test_batched_shallow_virtual_roots(cuda, filtered):
+    """Small query items (8-24 vertices, so G <= 32 lanes per query) placed across the root's
+    centre lines: their virtual root is the root or a node just below it, whose Partial-hazard
+    lists hold hundreds of containment candidates, scanned by the whole wave one query at a
+    time. Verdicts equal the oracle's with and without a hashed filter, under both stage
+    layouts."""
+    sc = S.config2_scene(seed=13, n_items=400)
+    cde, keys, oc = engines(sc)
+    box = sc.cde_bbox()
+    cx, cy = 0.5 * (box[0] + box[2]), 0.5 * (box[1] + box[3])
+    polys = [S.polygon(700 + i, 8, 24, 10.0, 30.0, 2) for i in range(3)]
+    prod, orc = items(polys)
+    rng = np.random.default_rng(5)
+    sets, expect, mats_all, first = [], [], [], 0
+    for i in range(len(polys)):
+        n = 2048
+        dt = np.empty((n, 3), dtype=np.float32)
+        dt[:, 0] = rng.uniform(0, 6.28, n)
+        across_x = rng.random(n) < 0.5                     # straddle the vertical or horizontal centre line
+        dt[:, 1] = np.where(across_x, cx + rng.uniform(-15, 15, n), rng.uniform(box[0], box[2], n))
+        dt[:, 2] = np.where(across_x, rng.uniform(box[1], box[3], n), cy + rng.uniform(-15, 15, n))
+        mats = J.compose(dt)
+        filt = J.HashedFilter(21 + i, 0.1) if filtered else None
+        of = O.make_filter(2, filt.seed, filt.threshold) if filt else None
+        sets.append((cde, prod[i], first, len(mats), filt))
+        expect.append(O.query(oc, orc[i], mats=mats, mode=1, filt=of, threads=8)["status"])
+        mats_all.append(mats)
+        first += len(mats)
+    ref = np.concatenate(expect)
+    assert 0 < (ref == 0).sum() and (ref == 1).sum() > 0
+    m = torch.from_numpy(np.concatenate(mats_all)).to(cuda)
+    for stages in (2, 1):
+        out = torch.full((first,), 9, dtype=torch.uint8, device=cuda)
+        J.QueryPlan(sets, stages=stages).detect(m, out)
+        np.testing.assert_array_equal(out.cpu().numpy(), ref)
