@@ -261,6 +261,97 @@ __device__ __forceinline__ bool scan_containment(const CdeView& c, const FilterV
 // the whole wave after the pass's other work, not by its own G lanes.
 constexpr uint32_t WAVE_SCAN_MIN = 64;
 
+// Sum of v over the 64 lanes (all active).
+__device__ __forceinline__ int wave_sum_i32(int v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// pip_hazard / pip_query with the wave's 64 lanes splitting the polygon's edges (lane l takes
+// edges l, l + 64, ...).  Each edge's crossing count is the sequential loop's, and their sum is
+// an integer, so the parity is the same.  All 64 lanes call them with the same arguments.
+__device__ __forceinline__ bool pip_hazard_wave(const CdeView& c, uint32_t slot, float px, float py, uint32_t lane) {
+    const float4 hb = c.haz_bbox[slot];
+    if (!box_has_point(hb, px, py)) return false;
+    const float ray_x = hb.z + (hb.z - hb.x);
+    const uint2 vr = c.haz_vr[slot];
+    const float2* v = c.haz_verts + vr.x;
+    int cnt = 0;
+    for (uint32_t i = lane; i < vr.y; i += 64u) {
+        const float2 a = v[i], b = v[i + 1 == vr.y ? 0 : i + 1];
+        cnt += pip_edge(px, py, ray_x, a.x, a.y, b.x, b.y);
+    }
+    return (wave_sum_i32(cnt) & 1) == 1;
+}
+__device__ __forceinline__ bool pip_query_wave(const ItemView& it, const Mat& m, float4 qbb, float px, float py,
+                                               uint32_t lane) {
+    if (!box_has_point(qbb, px, py)) return false;
+    const float ray_x = qbb.z + (qbb.z - qbb.x);
+    int cnt = 0;
+    for (uint32_t i = lane; i < it.n_v; i += 64u) {
+        const uint32_t k = i + 1 == it.n_v ? 0 : i + 1;
+        const float2 a = xform(m, it.v[i].x, it.v[i].y), b = xform(m, it.v[k].x, it.v[k].y);
+        cnt += pip_edge(px, py, ray_x, a.x, a.y, b.x, b.y);
+    }
+    return (wave_sum_i32(cnt) & 1) == 1;
+}
+
+// scan_containment for one query by the whole wave (all 64 lanes, the same arguments but
+// lane): candidates are filtered four per lane in flight; the point-in-polygon tests the
+// filter leaves (hazard bbox surrounding, or enclosed by, the query's) run on all 64 lanes,
+// since one lane walking a container of thousands of vertices would hold the wave.  The
+// result is the same: a candidate collides iff interior ? contained : !contained.
+template <bool FILT>
+__device__ __forceinline__ bool scan_containment_wave(const CdeView& c, const FilterView& f, uint64_t q, uint32_t e0,
+                                                      uint32_t cnt, uint32_t lane, const ItemView& it, const Mat& m,
+                                                      float4 bb, float2 qpoi) {
+    constexpr uint32_t CU = 4;
+    for (uint32_t k0 = 0; k0 < cnt; k0 += CU * 64u) {
+        uint32_t sl[CU];
+        float4 hb[CU];
+        float iw[CU];
+#pragma unroll
+        for (uint32_t r = 0; r < CU; r++) {
+            const uint32_t i = k0 + r * 64u + lane;
+            sl[r] = c.ent[e0 + (i < cnt ? i : 0u)].slot_flags & SLOT_MASK;
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < CU; r++) {
+            hb[r] = c.haz_bbox[sl[r]];
+            iw[r] = c.haz_poi[sl[r]].w;
+        }
+        bool direct = false;               // exterior scope, no bbox relation: !contained
+        uint32_t pend = 0;                 // bit r: candidate r needs its point-in-polygon test
+        int prel[CU];
+#pragma unroll
+        for (uint32_t r = 0; r < CU; r++) {
+            prel[r] = REL_DISJOINT;
+            if (k0 + r * 64u + lane >= cnt || irrel<FILT>(f, q, sl[r])) continue;
+            prel[r] = box_almost_relation(hb[r], bb);
+            if (prel[r] == REL_SURROUNDING || prel[r] == REL_ENCLOSED) pend |= 1u << r;
+            else if (iw[r] == 0.0f) direct = true;
+        }
+        if (ballot(direct) != 0ull) return true;
+#pragma unroll
+        for (uint32_t r = 0; r < CU; r++) {
+            for (uint64_t pm = ballot((pend >> r) & 1u); pm; pm &= pm - 1ull) {
+                const int l = __builtin_ctzll(pm);
+                const uint32_t s = (uint32_t)__shfl((int)sl[r], l, 64);
+                const int rl = __shfl(prel[r], l, 64);
+                bool contained;
+                if (rl == REL_SURROUNDING) {
+                    contained = pip_hazard_wave(c, s, qpoi.x, qpoi.y, lane);
+                } else {
+                    const float4 hp = c.haz_poi[s];
+                    contained = pip_query_wave(it, m, bb, hp.x, hp.y, lane);
+                }
+                if (c.haz_poi[s].w != 0.0f ? contained : !contained) return true;
+            }
+        }
+    }
+    return false;
+}
+
 // The polygon stage (cd_engine.rs:160-197) of the queries q0 + i for the set bits i of mask,
 // all of one set: 64/G queries per pass, G lanes each.
 template <bool FILT>
@@ -380,7 +471,7 @@ __device__ __forceinline__ void poly_queries(const SetDesc& sd, uint32_t q0, uin
             const uint64_t qq = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(q >> 32), l0, 64) << 32) |
                                 (uint32_t)__shfl((int)(uint32_t)q, l0, 64);
             const float2 pq = xform(mq, it.poi.x, it.poi.y);
-            const bool any = ballot(scan_containment<FILT>(c, f, qq, e0q, cq, lane, 64u, it, mq, bq, pq)) != 0ull;
+            const bool any = scan_containment_wave<FILT>(c, f, qq, e0q, cq, lane, it, mq, bq, pq);
             if ((int)(lane / G) == l0 / (int)G) hit = any;
         }
         const uint64_t bal = ballot(hit);
