@@ -215,24 +215,65 @@ __global__ __launch_bounds__(WAVE) void k_surrogate(const SetDesc* __restrict__ 
     }
 }
 
-// detect_containment_collision (cd_engine.rs:231-255; containment() inlined) over the
-// candidates i = start, start + stride, ... < cnt of the entry list at e0; true at the first
-// that collides.  Each lane keeps four candidates in flight (their entries, then their bboxes
-// and interior flags, read before any test): a shallow virtual root holds up to every hazard of
-// the layout, and one candidate at a time is a chain of dependent reads.
+// Sum of v over the G lanes of this lane's aligned group (G a power of two <= 64).
+__device__ __forceinline__ int group_sum_i32(int v, uint32_t G) {
+    for (uint32_t o = G >> 1; o > 0; o >>= 1) v += __shfl_xor(v, (int)o, 64);
+    return v;
+}
+
+// pip_hazard / pip_query with the G lanes of a group splitting the polygon's edges (lane sub
+// takes edges sub, sub + G, ...).  Each edge's crossing count is the sequential loop's, and
+// their sum is an integer, so the parity is the same.  The group's lanes call them together
+// with the same arguments.
+__device__ __forceinline__ bool pip_hazard_group(const CdeView& c, uint32_t slot, float px, float py, uint32_t sub,
+                                                 uint32_t G) {
+    const float4 hb = c.haz_bbox[slot];
+    if (!box_has_point(hb, px, py)) return false;
+    const float ray_x = hb.z + (hb.z - hb.x);
+    const uint2 vr = c.haz_vr[slot];
+    const float2* v = c.haz_verts + vr.x;
+    int cnt = 0;
+    for (uint32_t i = sub; i < vr.y; i += G) {
+        const float2 a = v[i], b = v[i + 1 == vr.y ? 0 : i + 1];
+        cnt += pip_edge(px, py, ray_x, a.x, a.y, b.x, b.y);
+    }
+    return (group_sum_i32(cnt, G) & 1) == 1;
+}
+__device__ __forceinline__ bool pip_query_group(const ItemView& it, const Mat& m, float4 qbb, float px, float py,
+                                                uint32_t sub, uint32_t G) {
+    if (!box_has_point(qbb, px, py)) return false;
+    const float ray_x = qbb.z + (qbb.z - qbb.x);
+    int cnt = 0;
+    for (uint32_t i = sub; i < it.n_v; i += G) {
+        const uint32_t k = i + 1 == it.n_v ? 0 : i + 1;
+        const float2 a = xform(m, it.v[i].x, it.v[i].y), b = xform(m, it.v[k].x, it.v[k].y);
+        cnt += pip_edge(px, py, ray_x, a.x, a.y, b.x, b.y);
+    }
+    return (group_sum_i32(cnt, G) & 1) == 1;
+}
+
+// detect_containment_collision (cd_engine.rs:231-255) over the candidates of one query, the
+// entries e0 .. e0 + cnt, by the query's G lanes (lane sub of the group starting at lane
+// base): true at the first that collides.  Each lane filters four candidates in flight (their
+// entries, then their bboxes and interior flags, read before any test): a shallow virtual root
+// holds up to every hazard of the layout.  A candidate without bbox relation collides iff its
+// scope is exterior; the point-in-polygon tests the filter leaves (hazard bbox surrounding, or
+// enclosed by, the query's) run on all G lanes, each polygon's edges split over them, since one
+// lane walking a container of thousands of vertices would hold the group.  The result is
+// containment()'s: interior ? contained : !contained, OR over the candidates.
 template <bool FILT>
 __device__ __forceinline__ bool scan_containment(const CdeView& c, const FilterView& f, uint64_t q, uint32_t e0,
-                                                 uint32_t cnt, uint32_t start, uint32_t stride, const ItemView& it,
-                                                 const Mat& m, float4 bb, float2 qpoi) {
+                                                 uint32_t cnt, uint32_t sub, uint32_t G, uint32_t base,
+                                                 const ItemView& it, const Mat& m, float4 bb, float2 qpoi) {
     constexpr uint32_t CU = 4;
-    bool hit = false;
-    for (uint32_t i0 = start; i0 < cnt && !hit; i0 += CU * stride) {
+    const uint64_t gmask = G >= 64u ? ~0ull : ((1ull << G) - 1ull);
+    for (uint32_t k0 = 0; k0 < cnt; k0 += CU * G) {
         uint32_t sl[CU];
         float4 hb[CU];
         float iw[CU];
 #pragma unroll
         for (uint32_t r = 0; r < CU; r++) {
-            const uint32_t i = i0 + r * stride;
+            const uint32_t i = k0 + r * G + sub;
             sl[r] = c.ent[e0 + (i < cnt ? i : 0u)].slot_flags & SLOT_MASK;
         }
 #pragma unroll
@@ -240,21 +281,36 @@ __device__ __forceinline__ bool scan_containment(const CdeView& c, const FilterV
             hb[r] = c.haz_bbox[sl[r]];
             iw[r] = c.haz_poi[sl[r]].w;
         }
+        bool direct = false;               // exterior scope, no bbox relation: !contained
+        uint32_t pend = 0;                 // bit r: candidate r needs its point-in-polygon test
+        int prel[CU];
 #pragma unroll
         for (uint32_t r = 0; r < CU; r++) {
-            if (hit || i0 + r * stride >= cnt || irrel<FILT>(f, q, sl[r])) continue;
-            const int rel = box_almost_relation(hb[r], bb);
-            bool contained = false;
-            if (rel == REL_SURROUNDING) {
-                contained = pip_hazard(c, sl[r], qpoi.x, qpoi.y);
-            } else if (rel == REL_ENCLOSED) {
-                const float4 hp = c.haz_poi[sl[r]];
-                contained = pip_query(it, m, bb, hp.x, hp.y);
+            prel[r] = REL_DISJOINT;
+            if (k0 + r * G + sub >= cnt || irrel<FILT>(f, q, sl[r])) continue;
+            prel[r] = box_almost_relation(hb[r], bb);
+            if (prel[r] == REL_SURROUNDING || prel[r] == REL_ENCLOSED) pend |= 1u << r;
+            else if (iw[r] == 0.0f) direct = true;
+        }
+        if ((ballot(direct) >> base) & gmask) return true;
+#pragma unroll
+        for (uint32_t r = 0; r < CU; r++) {
+            for (uint64_t pm = (ballot((pend >> r) & 1u) >> base) & gmask; pm; pm &= pm - 1ull) {
+                const int l = (int)base + __builtin_ctzll(pm);
+                const uint32_t s2 = (uint32_t)__shfl((int)sl[r], l, 64);
+                const int rl = __shfl(prel[r], l, 64);
+                bool contained;
+                if (rl == REL_SURROUNDING) {
+                    contained = pip_hazard_group(c, s2, qpoi.x, qpoi.y, sub, G);
+                } else {
+                    const float4 hp = c.haz_poi[s2];
+                    contained = pip_query_group(it, m, bb, hp.x, hp.y, sub, G);
+                }
+                if (c.haz_poi[s2].w != 0.0f ? contained : !contained) return true;
             }
-            if (iw[r] != 0.0f ? contained : !contained) hit = true;
         }
     }
-    return hit;
+    return false;
 }
 
 // A query whose virtual root holds more containment candidates than this has them scanned by
@@ -451,7 +507,7 @@ __device__ __forceinline__ void poly_queries(const SetDesc& sd, uint32_t q0, uin
                 heavy = true;
             } else {
                 const float2 qpoi = xform(m, it.poi.x, it.poi.y);
-                hit = scan_containment<FILT>(c, f, q, e0, cntv, sub, G, it, m, bb, qpoi);
+                hit = scan_containment<FILT>(c, f, q, e0, cntv, sub, G, slot * G, it, m, bb, qpoi);
             }
         }
         // the long candidate lists, one query at a time over all 64 lanes; the query's
