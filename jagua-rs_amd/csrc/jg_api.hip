@@ -422,15 +422,40 @@ __global__ __launch_bounds__(64) void k_collect(CdeView c, ItemView it, const fl
         }
         __syncthreads();
         {
+            // containment over the virtual root's Partial hazards not yet collected
+            // (cd_engine.rs:276-287): each lane filters one candidate by bbox relation (no
+            // relation: collides iff exterior scope); the point-in-polygon tests left run on
+            // all 64 lanes, each polygon's edges split over the wave (containment()'s result)
             const DevNode vn = c.nodes[vr];
             const float2 qpoi = xform(m, it.poi.x, it.poi.y);
-            const int he = (int)(vn.haz_begin + (vn.haz_info & COUNT_MASK));
-            for (int i = (int)vn.haz_begin + (int)lane; i < he; i += 64) {
-                const uint32_t sf = c.ent[i].slot_flags;
-                if (sf & ENTIRE_BIT) continue;
-                const uint32_t slot = sf & SLOT_MASK;
-                if (set_has(set, slot)) continue;
-                if (containment(c, slot, it, m, qbb, qpoi)) set_put(set, slot);
+            const uint32_t e0 = vn.haz_begin, cnt = vn.haz_info & COUNT_MASK;
+            for (uint32_t k0 = 0; k0 < cnt; k0 += 64u) {
+                const uint32_t i = k0 + lane;
+                uint32_t slot = 0;
+                int rel = REL_DISJOINT;
+                bool pend = false;
+                if (i < cnt) {
+                    const uint32_t sf = c.ent[e0 + i].slot_flags;
+                    slot = sf & SLOT_MASK;
+                    if (!(sf & ENTIRE_BIT) && !set_has(set, slot)) {
+                        rel = box_almost_relation(c.haz_bbox[slot], qbb);
+                        if (rel == REL_SURROUNDING || rel == REL_ENCLOSED) pend = true;
+                        else if (c.haz_poi[slot].w == 0.0f) set_put(set, slot);
+                    }
+                }
+                for (uint64_t pm = __ballot(pend); pm; pm &= pm - 1ull) {
+                    const int l = __builtin_ctzll(pm);
+                    const uint32_t s = (uint32_t)__shfl((int)slot, l, 64);
+                    const int rl = __shfl(rel, l, 64);
+                    bool contained;
+                    if (rl == REL_SURROUNDING) {
+                        contained = pip_hazard_wave(c, s, qpoi.x, qpoi.y, lane);
+                    } else {
+                        const float4 hp = c.haz_poi[s];
+                        contained = pip_query_wave(it, m, qbb, hp.x, hp.y, lane);
+                    }
+                    if (lane == (uint32_t)l && (c.haz_poi[s].w != 0.0f ? contained : !contained)) set_put(set, s);
+                }
             }
         }
         __syncthreads();

@@ -317,41 +317,6 @@ __device__ __forceinline__ bool scan_containment(const CdeView& c, const FilterV
 // the whole wave after the pass's other work, not by its own G lanes.
 constexpr uint32_t WAVE_SCAN_MIN = 64;
 
-// Sum of v over the 64 lanes (all active).
-__device__ __forceinline__ int wave_sum_i32(int v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// pip_hazard / pip_query with the wave's 64 lanes splitting the polygon's edges (lane l takes
-// edges l, l + 64, ...).  Each edge's crossing count is the sequential loop's, and their sum is
-// an integer, so the parity is the same.  All 64 lanes call them with the same arguments.
-__device__ __forceinline__ bool pip_hazard_wave(const CdeView& c, uint32_t slot, float px, float py, uint32_t lane) {
-    const float4 hb = c.haz_bbox[slot];
-    if (!box_has_point(hb, px, py)) return false;
-    const float ray_x = hb.z + (hb.z - hb.x);
-    const uint2 vr = c.haz_vr[slot];
-    const float2* v = c.haz_verts + vr.x;
-    int cnt = 0;
-    for (uint32_t i = lane; i < vr.y; i += 64u) {
-        const float2 a = v[i], b = v[i + 1 == vr.y ? 0 : i + 1];
-        cnt += pip_edge(px, py, ray_x, a.x, a.y, b.x, b.y);
-    }
-    return (wave_sum_i32(cnt) & 1) == 1;
-}
-__device__ __forceinline__ bool pip_query_wave(const ItemView& it, const Mat& m, float4 qbb, float px, float py,
-                                               uint32_t lane) {
-    if (!box_has_point(qbb, px, py)) return false;
-    const float ray_x = qbb.z + (qbb.z - qbb.x);
-    int cnt = 0;
-    for (uint32_t i = lane; i < it.n_v; i += 64u) {
-        const uint32_t k = i + 1 == it.n_v ? 0 : i + 1;
-        const float2 a = xform(m, it.v[i].x, it.v[i].y), b = xform(m, it.v[k].x, it.v[k].y);
-        cnt += pip_edge(px, py, ray_x, a.x, a.y, b.x, b.y);
-    }
-    return (wave_sum_i32(cnt) & 1) == 1;
-}
-
 // scan_containment for one query by the whole wave (all 64 lanes, the same arguments but
 // lane): candidates are filtered four per lane in flight; the point-in-polygon tests the
 // filter leaves (hazard bbox surrounding, or enclosed by, the query's) run on all 64 lanes,
