@@ -262,6 +262,107 @@ def test_edge_quadrants_superset():
                 assert mask >> k & 1, (e, k, mask)
 
 
+def quad_mask(e, r):
+    a, pa = arr(*e)
+    b, pb = arr(*r)
+    m = L.or_edge_quadrants(pa, pb)
+    return [m >> k & 1 for k in range(4)]
+
+
+def test_edge_quadrants_hand_derived():
+    """Masks of Edge::collides_with_quadrants (qt_traits.rs:19-106) worked out by hand on
+    r = [0,4]^2: quadrants q0 [2,4]x[2,4], q1 [0,2]x[2,4], q2 [0,2]^2, q3 [2,4]x[0,2]; sides
+    top (4,4)->(0,4), left (0,4)->(0,0), bottom (0,0)->(4,0), right (4,0)->(4,4); bisectors
+    h (0,2)->(4,2), v (2,0)->(2,4).  In edge_intersection_half u = 1 at e2's start, so u > 0.5
+    is e2's first half."""
+    r = (0.0, 0.0, 4.0, 4.0)
+    # Vertical edge on the v bisector: all four bbox tests overlap (x = 2 is in every
+    # quadrant's x range), no end point inside.  Left, right and v are parallel (no
+    # intersection).  Top is crossed at (2, 4): u = 0.5, Halfway, sets q0 and q1; bottom at
+    # (2, 0): Halfway, sets q2 and q3.  Without the Halfway arm q3 would stay unset.
+    assert quad_mask((2, -1, 2, 5), r) == [1, 1, 1, 1]
+    # The same on the h bisector: left crossed at (0, 2) and right at (4, 2), both Halfway.
+    assert quad_mask((-1, 2, 5, 2), r) == [1, 1, 1, 1]
+    # Vertical at x = 3: q1 and q2 fail the bbox test (3 > 2); top at (3, 4) has u = 0.75
+    # (first half: q0), bottom at (3, 0) u = 0.25 (second half: q3).
+    assert quad_mask((3, 5, 3, -1), r) == [1, 0, 0, 1]
+    # Through the top-left corner only: q0, q2, q3 fail the bbox test; left is crossed at its
+    # start (0, 4), u = 1: first half, q1.  The edge touches q1 in one point (rects are closed).
+    assert quad_mask((-1, 3, 1, 5), r) == [0, 1, 0, 0]
+    # The diagonal through the centre: corners (0,0) and (4,4) set q2 and q0 through left,
+    # right, bottom; the h bisector is crossed at its midpoint (u = -12 / -24 = 0.5 exactly),
+    # Halfway: q1 and q3 as well (they touch the edge at (2, 2)).
+    assert quad_mask((-1, -1, 5, 5), r) == [1, 1, 1, 1]
+
+
+def test_edge_quadrants_rounding_overwrite():
+    """A flag that the bbox test set to Some(false) is overwritten with Some(true) by a later
+    half-intersection (qt_traits.rs:110-140: the guard only asks that SOME target is unset).
+    The edge from (-1, a) to (5, b), a = 2 - 2^-22, b = 2 - 2^-23, lies below y = 2, so q0
+    and q1 fail the bbox test (Some(false)); q2 and q3 are unset.  The right side call
+    (targets q3, q0) computes y2 - y4 = b - 4 = -2 - 2^-23, which rounds (tie to even) to -2,
+    so u_nom = (x2 - x4)(y2 - y1) - (y2 - y4)(x2 - x1) = 1 * 2^-23 + 12 rounds to 12, and
+    u = 12 / 24 = 0.5 exactly: Halfway sets q3 AND q0.  The left call (targets q1, q2) gives
+    u = 0.5 - 3 * 2^-25 (second half: q2 only).  So the reference reports q0 although the edge
+    does not touch it; an implementation that only filled unset flags would not."""
+    a, b = 2 - 2 ** -22, 2 - 2 ** -23
+    assert float(f32(a)) == a and float(f32(b)) == b and float(f32(b - 4)) == -2.0
+    assert quad_mask((-1.0, a, 5.0, b), (0.0, 0.0, 4.0, 4.0)) == [1, 0, 1, 1]
+    # the same along x: the top call (targets q0, q1) stays in the second half for this edge,
+    # the bottom call (targets q2, q3) rounds to Halfway and sets q3
+    assert quad_mask((a, 5.0, b, -1.0), (0.0, 0.0, 4.0, 4.0)) == [0, 1, 1, 1]
+
+
+def exact_seg_meets_rect(e, q, inset):
+    """Exact (rationals): does the closed segment e meet the closed rectangle q shrunk by
+    `inset` on every side (inset < 0 grows it)?  Liang-Barsky clipping."""
+    x1, y1, x2, y2 = map(Fr, e)
+    lo_x, lo_y, hi_x, hi_y = Fr(q[0]) + inset, Fr(q[1]) + inset, Fr(q[2]) - inset, Fr(q[3]) - inset
+    t0, t1 = Fr(0), Fr(1)
+    for p, d in ((-(x2 - x1), x1 - lo_x), (x2 - x1, hi_x - x1), (-(y2 - y1), y1 - lo_y), (y2 - y1, hi_y - y1)):
+        if p == 0:
+            if d < 0:
+                return False
+            continue
+        t = d / p
+        if p < 0:
+            t0 = max(t0, t)
+        else:
+            t1 = min(t1, t)
+        if t0 > t1:
+            return False
+    return True
+
+
+def test_edge_quadrants_equal_exact_masks():
+    """Away from rounding, the mask is exactly the set of quadrants the edge touches (not a
+    superset): for edges that either pass through a quadrant's interior by a margin or stay a
+    margin away from it, for every quadrant, the oracle's mask equals the exact one."""
+    rng = random.Random(8)
+    r = (0.0, 0.0, 64.0, 64.0)
+    qs = [(32, 32, 64, 64), (0, 32, 32, 64), (0, 0, 32, 32), (32, 0, 64, 32)]
+    eps = Fr(1, 1 << 10)
+    checked = 0
+    for _ in range(6000):
+        e = [rnd32(rng, -10, 74) for _ in range(4)]
+        if rng.random() < 0.3:                        # short edges inside and across quadrants
+            e[2], e[3] = float(f32(e[0] + rng.uniform(-12, 12))), float(f32(e[1] + rng.uniform(-12, 12)))
+        exact = []
+        for q in qs:
+            if exact_seg_meets_rect(e, q, eps):
+                exact.append(1)
+            elif not exact_seg_meets_rect(e, q, -eps):
+                exact.append(0)
+            else:
+                exact = None
+                break
+        if exact is None:
+            continue
+        checked += 1
+        assert quad_mask(e, r) == exact, (e, exact)
+    assert checked > 5500
+
+
 # ------------------------------------------------------------------------ transformation
 def test_transform_matrix_closed_form():
     dt = np.array([[0.0, 3.0, 4.0], [math.pi / 2, 0, 0], [1.234, -5, 6]], dtype=np.float32)
