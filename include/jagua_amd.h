@@ -269,6 +269,17 @@ jg_status jg_plan_detect(jg_plan* plan, const float* d_mat6, uint8_t* d_status, 
 /* The same launch captured once in a hipGraph and replayed (launch overhead amortised). */
 jg_status jg_plan_detect_graph(jg_plan* plan, const float* d_mat6, uint8_t* d_status, uint32_t replays,
                                void* stream);
+/* The hazard set of every query of the plan, as jg_collect_batch computes it for one set
+ * (collect_surrogate_collisions ∪ collect_poly_collisions, cd_engine.rs:261-310, composed as in
+ * lbf/benches/ci_bench.rs:56-69), for all sets in one persistent launch.  Outputs are indexed
+ * like d_status: d_count[i], d_hash[i] and, when d_keys is not NULL, the first `capacity` slots
+ * ascending in d_keys[i*capacity..] (the rest 0xffffffff).  The collect APIs of the reference
+ * take no filter; a set's JG_FILTER_KEYS filter is read as a pre-seeded collector instead (the
+ * collector is the filter, filter.rs:69-76): every query's set starts with those keys and the
+ * result is the seed together with the hits.  A JG_FILTER_HASHED filter is rejected.  Every
+ * item needs its surrogate (its ff poles are collected). */
+jg_status jg_plan_collect(jg_plan* plan, const float* d_m6, uint32_t capacity, uint32_t* d_count, uint64_t* d_hash,
+                          uint32_t* d_keys, void* stream);
 /* Schedule autotuner (no reference counterpart: the reference evaluates one query at a
  * time).  Times the plan's launch on `stream` with HIP events for candidate schedules —
  * stage layout {two kernels, one kernel}, lanes per query in the polygon stage {per item, 8,

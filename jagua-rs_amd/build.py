@@ -38,7 +38,7 @@ DEVICE = [
     "-fno-gpu-flush-denormals-to-zero", "-munsafe-fp-atomics",
 ]
 
-SOURCES = ["jg_host.cpp", "jg_api.hip", "jg_batched.hip", "jg_overlap.hip", "jg_poles.hip", "jg_search.hip", "jg_piers.hip"]
+SOURCES = ["jg_host.cpp", "jg_api.hip", "jg_batched.hip", "jg_overlap.hip", "jg_poles.hip", "jg_search.hip", "jg_piers.hip", "jg_collect.hip"]
 
 
 def _run(cmd):

@@ -12,6 +12,7 @@
 #include "jg_device.hpp"
 #include "jg_host.hpp"
 #include "jg_internal.hpp"
+#include "jg_plan.hpp"
 
 using namespace jg;
 
@@ -70,32 +71,43 @@ jg_status device_ready() {
     return JG_OK;
 }
 
+size_t item_image_bytes(const Polygon& p) {
+    const uint32_t n_ff = p.sur.present ? std::min<uint32_t>(p.sur.cfg.n_ff_poles, (uint32_t)p.sur.poles.size()) : 0;
+    const uint32_t n_pi = p.sur.present ? (uint32_t)p.sur.piers.size() : 0;
+    return align256(align256(sizeof(float2) * p.n()) + sizeof(float4) * (n_ff + n_pi + 1));
+}
+
+// The device record of a query item (vertices, then the ff poles and piers) written into
+// `host`, whose bytes are copied to device address `dev`; `view` points at `dev`.
+void item_image_write(const Polygon& p, uint8_t* host, uint8_t* dev, ItemView& view) {
+    const uint32_t n_ff = p.sur.present ? std::min<uint32_t>(p.sur.cfg.n_ff_poles, (uint32_t)p.sur.poles.size()) : 0;
+    const uint32_t n_pi = p.sur.present ? (uint32_t)p.sur.piers.size() : 0;
+    const size_t off_ff = align256(sizeof(float2) * p.n());
+    std::memset(host, 0, item_image_bytes(p));
+    std::memcpy(host, p.v.data(), sizeof(float2) * p.n());
+    float4* ff = (float4*)(host + off_ff);
+    for (uint32_t i = 0; i < n_ff; i++) ff[i] = p.sur.poles[i];
+    for (uint32_t i = 0; i < n_pi; i++) ff[n_ff + i] = p.sur.piers[i];
+    view.v = (const float2*)dev;
+    view.ff = (const float4*)(dev + off_ff);
+    view.poi = p.poi;
+    view.n_v = p.n();
+    view.n_poles = n_ff;
+    view.n_piers = n_pi;
+}
+
 jg_status item_upload(jg_spolygon* it) {
     if (it->dev_valid) return JG_OK;
     jg_status s = device_ready();
     if (s != JG_OK) return s;
-    const Polygon& p = it->poly;
-    const uint32_t n_ff = p.sur.present ? std::min<uint32_t>(p.sur.cfg.n_ff_poles, (uint32_t)p.sur.poles.size()) : 0;
-    const uint32_t n_pi = p.sur.present ? (uint32_t)p.sur.piers.size() : 0;
-    const size_t off_v = 0;
-    const size_t off_ff = align256(off_v + sizeof(float2) * p.n());
-    const size_t total = align256(off_ff + sizeof(float4) * (n_ff + n_pi + 1));
+    const size_t total = item_image_bytes(it->poly);
     std::vector<uint8_t> host(total, 0);
-    std::memcpy(host.data() + off_v, p.v.data(), sizeof(float2) * p.n());
-    float4* ff = (float4*)(host.data() + off_ff);
-    for (uint32_t i = 0; i < n_ff; i++) ff[i] = p.sur.poles[i];
-    for (uint32_t i = 0; i < n_pi; i++) ff[n_ff + i] = p.sur.piers[i];
     s = it->dev.reserve(total);
     if (s != JG_OK) return s;
+    item_image_write(it->poly, host.data(), (uint8_t*)it->dev.ptr, it->view);
     HIP_TRY(hipMemcpy(it->dev.ptr, host.data(), total, hipMemcpyHostToDevice));
-    uint8_t* base = (uint8_t*)it->dev.ptr;
-    it->view.v = (const float2*)(base + off_v);
-    it->view.ff = (const float4*)(base + off_ff);
-    it->view.poi = p.poi;
-    it->view.n_v = p.n();
-    it->view.n_poles = n_ff;
-    it->view.n_piers = n_pi;
     it->dev_valid = true;
+    it->upload_gen++;
     return JG_OK;
 }
 
@@ -127,6 +139,16 @@ static void set_view(jg_cde* c) {
     v.exterior_slot = c->exterior.idx;
     v.cd_threshold = c->cfg.cd_threshold;
     v.n_nodes = (uint32_t)im.nodes.size();
+    v.ext_inner = make_float4(1.0f, 1.0f, 0.0f, 0.0f);
+    if (c->map.valid(c->exterior)) {
+        const Polygon& ex = c->map.slots[c->exterior.idx].rec.shape;
+        bool rect = ex.n() == 4;
+        for (uint32_t i = 0; i < 4 && rect; i++) {
+            const float4 e = ex.edge(i);
+            rect = e.x == e.z || e.y == e.w;
+        }
+        if (rect) v.ext_inner = ex.bbox;
+    }
     c->stats.n_nodes = im.nodes.size();
     c->stats.n_entries = im.ent.size();
     c->stats.n_edge_refs = im.edges.size();
@@ -214,8 +236,11 @@ jg_status cde_upload(jg_cde* c) {
     return JG_OK;
 }
 
-// JG_FILTER_KEYS masks are uploaded into a per-engine scratch buffer.
-jg_status filter_view(jg_cde* c, const jg_hazard_filter* f, FilterView& out) {
+// JG_FILTER_KEYS masks are uploaded into a per-engine buffer.  The launch that last read it may
+// still run on another stream (a non-blocking one does not wait for the null stream), so the
+// buffer is overwritten only after that launch's completion event; the copy is then ordered on
+// the launch stream.  filter_read_done records the event after a launch that reads the mask.
+jg_status filter_view(jg_cde* c, const jg_hazard_filter* f, FilterView& out, hipStream_t stream) {
     out = FilterView{nullptr, 0, 0, 0, 0};
     if (!f || f->kind == JG_FILTER_NONE) return JG_OK;
     if (f->kind == JG_FILTER_HASHED) {
@@ -231,12 +256,20 @@ jg_status filter_view(jg_cde* c, const jg_hazard_filter* f, FilterView& out) {
         const jg_hazkey k = f->keys[i];
         if (c->map.valid(k)) mask[k.idx >> 5] |= 1u << (k.idx & 31u);
     }
+    if (c->filter_read) HIP_TRY(hipEventSynchronize(c->filter_read));
     jg_status s = c->filter_dev.reserve(sizeof(uint32_t) * words);
     if (s != JG_OK) return s;
-    HIP_TRY(hipMemcpy(c->filter_dev.ptr, mask.data(), sizeof(uint32_t) * words, hipMemcpyHostToDevice));
+    // pageable source: the copy is complete when the call returns, and ordered on `stream`
+    HIP_TRY(hipMemcpyAsync(c->filter_dev.ptr, mask.data(), sizeof(uint32_t) * words, hipMemcpyHostToDevice, stream));
     out.kind = 1;
     out.mask = (const uint32_t*)c->filter_dev.ptr;
     out.n_words = words;
+    return JG_OK;
+}
+
+jg_status filter_read_done(jg_cde* c, hipStream_t stream) {
+    if (!c->filter_read) HIP_TRY(hipEventCreateWithFlags(&c->filter_read, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(c->filter_read, stream));
     return JG_OK;
 }
 
@@ -374,14 +407,149 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
     return v;
 }
 
+// One hazard-collection query (ci_bench.rs:56-69: collect_surrogate_collisions then
+// collect_poly_collisions, cd_engine.rs:261-310) by one wavefront, whose workgroup is that
+// wavefront alone, so that the workgroup barrier orders the lanes' LDS accesses.  The collector
+// is a SlotSet in LDS (union semantics of BasicHazardCollector, collector.rs:44-67), empty
+// between queries; it starts with the n_seed slots of `seed` (a pre-seeded collector: the
+// result is the seed together with the hits).  Lanes take the surrogate poles/piers and the
+// query edges, each collecting its descent (collect_from); containment checks follow.  The
+// count, the order-independent hash and the first `cap` slots ascending are written to
+// count[out], hash[out], keys[out * cap ..].
+__device__ __forceinline__ void collect_query(const CdeView& c, const ItemView& it, const Mat& m, uint64_t out,
+                                              const uint32_t* seed, uint32_t n_seed, uint32_t words, uint32_t cap,
+                                              uint32_t* lds, uint32_t* list, uint32_t* list_n,
+                                              uint32_t* __restrict__ count, uint64_t* __restrict__ hash,
+                                              uint32_t* __restrict__ keys, uint32_t lane) {
+    const SlotSet set{lds, list, list_n};
+    for (uint32_t i = lane; i < n_seed; i += 64) set_put(set, seed[i]);
+    const uint32_t n_sur = it.n_poles + it.n_piers;
+    for (uint32_t i = lane; i < n_sur; i += 64) {
+        const float4 p = it.ff[i];
+        if (i < it.n_poles) {
+            const float2 cc = xform(m, p.x, p.y);
+            collect_from<true>(c, 0, make_float4(cc.x, cc.y, 0, 0), p.z, set);
+        } else {
+            const float2 a = xform(m, p.x, p.y), b = xform(m, p.z, p.w);
+            collect_from<false>(c, 0, make_float4(a.x, a.y, b.x, b.y), 0.0f, set);
+        }
+    }
+    float4 bb = make_float4(3.40282347e+38f, 3.40282347e+38f, -3.40282347e+38f, -3.40282347e+38f);
+    for (uint32_t i = lane; i < it.n_v; i += 64) {
+        const float2 p = xform(m, it.v[i].x, it.v[i].y);
+        bb.x = mn(bb.x, p.x); bb.y = mn(bb.y, p.y); bb.z = mx(bb.z, p.x); bb.w = mx(bb.w, p.y);
+    }
+    const float4 qbb = make_float4(wave_min(bb.x), wave_min(bb.y), wave_max(bb.z), wave_max(bb.w));
+    if (box_relation(c.nodes[0].bbox, qbb) != REL_SURROUNDING && lane == 0) set_put(set, c.exterior_slot);
+    const int vr = virtual_root(c, qbb);
+    for (uint32_t i = lane; i < it.n_v; i += 64) {
+        const float2 a = xform(m, it.v[i].x, it.v[i].y);
+        const uint32_t j = i + 1 == it.n_v ? 0 : i + 1;
+        const float2 b = xform(m, it.v[j].x, it.v[j].y);
+        collect_from<false>(c, vr, make_float4(a.x, a.y, b.x, b.y), 0.0f, set);
+    }
+    __syncthreads();
+    {
+        // containment over the virtual root's Partial hazards not yet collected
+        // (cd_engine.rs:276-287): each lane filters one candidate by bbox relation (no
+        // relation: collides iff exterior scope); the point-in-polygon tests left run on
+        // all 64 lanes, each polygon's edges split over the wave (containment()'s result)
+        const DevNode vn = c.nodes[vr];
+        const float2 qpoi = xform(m, it.poi.x, it.poi.y);
+        const uint32_t e0 = vn.haz_begin, cnt = vn.haz_info & COUNT_MASK;
+        for (uint32_t k0 = 0; k0 < cnt; k0 += 64u) {
+            const uint32_t i = k0 + lane;
+            uint32_t slot = 0;
+            int rel = REL_DISJOINT;
+            bool pend = false;
+            if (i < cnt) {
+                const uint32_t sf = c.ent[e0 + i].slot_flags;
+                slot = sf & SLOT_MASK;
+                if (!(sf & ENTIRE_BIT) && !set_has(set, slot)) {
+                    rel = box_almost_relation(c.haz_bbox[slot], qbb);
+                    if (rel == REL_SURROUNDING || rel == REL_ENCLOSED) pend = true;
+                    else if (c.haz_poi[slot].w == 0.0f) set_put(set, slot);
+                }
+            }
+            for (uint64_t pm = __ballot(pend); pm; pm &= pm - 1ull) {
+                const int l = __builtin_ctzll(pm);
+                const uint32_t s = (uint32_t)__shfl((int)slot, l, 64);
+                const int rl = __shfl(rel, l, 64);
+                bool contained;
+                if (rl == REL_SURROUNDING) {
+                    contained = pip_hazard_wave(c, s, qpoi.x, qpoi.y, lane);
+                } else {
+                    const float4 hp = c.haz_poi[s];
+                    contained = pip_query_wave(it, m, qbb, hp.x, hp.y, lane);
+                }
+                if (lane == (uint32_t)l && (c.haz_poi[s].w != 0.0f ? contained : !contained)) set_put(set, s);
+            }
+        }
+    }
+    __syncthreads();
+    // results: count, order-independent hash, the first `cap` slots ascending
+    const uint32_t total = *list_n;
+    uint32_t cnt = 0;
+    uint64_t h = 0;
+    if (total <= SET_LIST_CAP) {
+        for (uint32_t i = lane; i < total; i += 64) {
+            const uint32_t si = list[i];
+            h += mix64((uint64_t)si);
+            if (keys && cap) {
+                uint32_t rank = 0;
+                for (uint32_t j = 0; j < total; j++) rank += list[j] < si ? 1u : 0u;
+                if (rank < cap) keys[out * cap + rank] = si;
+            }
+        }
+        cnt = total;
+        if (keys)
+            for (uint32_t k = total + lane; k < cap; k += 64) keys[out * cap + k] = 0xffffffffu;
+    } else {
+        for (uint32_t w = lane; w < words; w += 64) {
+            uint32_t bits = lds[w];
+            cnt += (uint32_t)__builtin_popcount(bits);
+            while (bits) {
+                const uint32_t b = (uint32_t)__builtin_ctz(bits);
+                bits &= bits - 1;
+                h += mix64((uint64_t)(w * 32u + b));
+            }
+        }
+        cnt = wave_sum_u32(cnt);
+        if (lane == 0 && keys && cap) {
+            uint32_t k = 0;
+            for (uint32_t w = 0; w < words && k < cap; w++) {
+                uint32_t bits = lds[w];
+                while (bits && k < cap) {
+                    const uint32_t b = (uint32_t)__builtin_ctz(bits);
+                    bits &= bits - 1;
+                    keys[out * cap + k++] = w * 32u + b;
+                }
+            }
+            for (; k < cap; k++) keys[out * cap + k] = 0xffffffffu;
+        }
+    }
+    h = wave_sum_u64(h);
+    if (lane == 0) {
+        count[out] = cnt;
+        hash[out] = h;
+    }
+    __syncthreads();
+    // reset the collector for the next query
+    if (total <= SET_LIST_CAP) {
+        for (uint32_t i = lane; i < total; i += 64) lds[list[i] >> 5] = 0u;
+    } else {
+        for (uint32_t w = lane; w < words; w += 64) lds[w] = 0u;
+    }
+    if (lane == 0) *list_n = 0u;
+    __syncthreads();
+}
+
 }  // namespace
 
-// Hazard-collection queries, one at a time per wavefront (persistent one-wavefront workgroups,
-// so the workgroup barrier orders the lanes' LDS accesses); the collector is a SlotSet in LDS
-// (union semantics of BasicHazardCollector, collector.rs:44-67).  Lanes take the surrogate
-// poles/piers and the query edges, each collecting its descent (collect_from); containment
-// checks follow (cd_engine.rs:261-310).  A wave traversal with shared queues, as the batched
-// detection uses, measured slower here: collection descents start at deep virtual roots.
+// Hazard-collection queries of one item, one at a time per wavefront (persistent one-wavefront
+// workgroups).  A wave traversal with shared queues, as the batched forms use, measured slower
+// here without the subtree skip of the batched collection: collection descents start at deep
+// virtual roots.
 __global__ __launch_bounds__(64) void k_collect(CdeView c, ItemView it, const float* __restrict__ m6, uint64_t n,
                                                 uint32_t words, uint32_t cap, uint32_t* __restrict__ count,
                                                 uint64_t* __restrict__ hash, uint32_t* __restrict__ keys) {
@@ -389,133 +557,47 @@ __global__ __launch_bounds__(64) void k_collect(CdeView c, ItemView it, const fl
     __shared__ uint32_t list[SET_LIST_CAP];
     __shared__ uint32_t list_n;
     const uint32_t lane = threadIdx.x;
-    const SlotSet set{lds, list, &list_n};
     for (uint32_t w = lane; w < words; w += 64) lds[w] = 0u;
     if (lane == 0) list_n = 0u;
     __syncthreads();
-    for (uint64_t q = blockIdx.x; q < n; q += gridDim.x) {
-        const Mat m = load_mat(m6, q);
-        const uint32_t n_sur = it.n_poles + it.n_piers;
-        for (uint32_t i = lane; i < n_sur; i += 64) {
-            const float4 p = it.ff[i];
-            if (i < it.n_poles) {
-                const float2 cc = xform(m, p.x, p.y);
-                collect_from<true>(c, 0, make_float4(cc.x, cc.y, 0, 0), p.z, set);
-            } else {
-                const float2 a = xform(m, p.x, p.y), b = xform(m, p.z, p.w);
-                collect_from<false>(c, 0, make_float4(a.x, a.y, b.x, b.y), 0.0f, set);
-            }
-        }
-        float4 bb = make_float4(3.40282347e+38f, 3.40282347e+38f, -3.40282347e+38f, -3.40282347e+38f);
-        for (uint32_t i = lane; i < it.n_v; i += 64) {
-            const float2 p = xform(m, it.v[i].x, it.v[i].y);
-            bb.x = mn(bb.x, p.x); bb.y = mn(bb.y, p.y); bb.z = mx(bb.z, p.x); bb.w = mx(bb.w, p.y);
-        }
-        const float4 qbb = make_float4(wave_min(bb.x), wave_min(bb.y), wave_max(bb.z), wave_max(bb.w));
-        if (box_relation(c.nodes[0].bbox, qbb) != REL_SURROUNDING && lane == 0) set_put(set, c.exterior_slot);
-        const int vr = virtual_root(c, qbb);
-        for (uint32_t i = lane; i < it.n_v; i += 64) {
-            const float2 a = xform(m, it.v[i].x, it.v[i].y);
-            const uint32_t j = i + 1 == it.n_v ? 0 : i + 1;
-            const float2 b = xform(m, it.v[j].x, it.v[j].y);
-            collect_from<false>(c, vr, make_float4(a.x, a.y, b.x, b.y), 0.0f, set);
-        }
-        __syncthreads();
-        {
-            // containment over the virtual root's Partial hazards not yet collected
-            // (cd_engine.rs:276-287): each lane filters one candidate by bbox relation (no
-            // relation: collides iff exterior scope); the point-in-polygon tests left run on
-            // all 64 lanes, each polygon's edges split over the wave (containment()'s result)
-            const DevNode vn = c.nodes[vr];
-            const float2 qpoi = xform(m, it.poi.x, it.poi.y);
-            const uint32_t e0 = vn.haz_begin, cnt = vn.haz_info & COUNT_MASK;
-            for (uint32_t k0 = 0; k0 < cnt; k0 += 64u) {
-                const uint32_t i = k0 + lane;
-                uint32_t slot = 0;
-                int rel = REL_DISJOINT;
-                bool pend = false;
-                if (i < cnt) {
-                    const uint32_t sf = c.ent[e0 + i].slot_flags;
-                    slot = sf & SLOT_MASK;
-                    if (!(sf & ENTIRE_BIT) && !set_has(set, slot)) {
-                        rel = box_almost_relation(c.haz_bbox[slot], qbb);
-                        if (rel == REL_SURROUNDING || rel == REL_ENCLOSED) pend = true;
-                        else if (c.haz_poi[slot].w == 0.0f) set_put(set, slot);
-                    }
-                }
-                for (uint64_t pm = __ballot(pend); pm; pm &= pm - 1ull) {
-                    const int l = __builtin_ctzll(pm);
-                    const uint32_t s = (uint32_t)__shfl((int)slot, l, 64);
-                    const int rl = __shfl(rel, l, 64);
-                    bool contained;
-                    if (rl == REL_SURROUNDING) {
-                        contained = pip_hazard_wave(c, s, qpoi.x, qpoi.y, lane);
-                    } else {
-                        const float4 hp = c.haz_poi[s];
-                        contained = pip_query_wave(it, m, qbb, hp.x, hp.y, lane);
-                    }
-                    if (lane == (uint32_t)l && (c.haz_poi[s].w != 0.0f ? contained : !contained)) set_put(set, s);
-                }
-            }
-        }
-        __syncthreads();
-        // results: count, order-independent hash, the first `cap` slots ascending
-        const uint32_t total = list_n;
-        uint32_t cnt = 0;
-        uint64_t h = 0;
-        if (total <= SET_LIST_CAP) {
-            for (uint32_t i = lane; i < total; i += 64) {
-                const uint32_t si = list[i];
-                h += mix64((uint64_t)si);
-                if (keys && cap) {
-                    uint32_t rank = 0;
-                    for (uint32_t j = 0; j < total; j++) rank += list[j] < si ? 1u : 0u;
-                    if (rank < cap) keys[q * cap + rank] = si;
-                }
-            }
-            cnt = total;
-            if (keys)
-                for (uint32_t k = total + lane; k < cap; k += 64) keys[q * cap + k] = 0xffffffffu;
-        } else {
-            for (uint32_t w = lane; w < words; w += 64) {
-                uint32_t bits = lds[w];
-                cnt += (uint32_t)__builtin_popcount(bits);
-                while (bits) {
-                    const uint32_t b = (uint32_t)__builtin_ctz(bits);
-                    bits &= bits - 1;
-                    h += mix64((uint64_t)(w * 32u + b));
-                }
-            }
-            cnt = wave_sum_u32(cnt);
-            if (lane == 0 && keys && cap) {
-                uint32_t k = 0;
-                for (uint32_t w = 0; w < words && k < cap; w++) {
-                    uint32_t bits = lds[w];
-                    while (bits && k < cap) {
-                        const uint32_t b = (uint32_t)__builtin_ctz(bits);
-                        bits &= bits - 1;
-                        keys[q * cap + k++] = w * 32u + b;
-                    }
-                }
-                for (; k < cap; k++) keys[q * cap + k] = 0xffffffffu;
-            }
-        }
-        h = wave_sum_u64(h);
-        if (lane == 0) {
-            count[q] = cnt;
-            hash[q] = h;
-        }
-        __syncthreads();
-        // reset the collector for the next query
-        if (total <= SET_LIST_CAP) {
-            for (uint32_t i = lane; i < total; i += 64) lds[list[i] >> 5] = 0u;
-        } else {
-            for (uint32_t w = lane; w < words; w += 64) lds[w] = 0u;
-        }
-        if (lane == 0) list_n = 0u;
-        __syncthreads();
+    for (uint64_t q = blockIdx.x; q < n; q += gridDim.x)
+        collect_query(c, it, load_mat(m6, q), q, nullptr, 0u, words, cap, lds, list, &list_n, count, hash, keys, lane);
+}
+
+// The queries the batched collection could not hold in its per-query collectors (more slots
+// than it keeps, counted at counters[C_OVERFLOW]): (set << 32 | query in set) entries of ovf,
+// each collected here by one wave with the per-query collector over every slot.
+__global__ __launch_bounds__(64) void k_collect_overflow(const SetDesc* __restrict__ sets,
+                                                         const uint32_t* __restrict__ counters,
+                                                         const uint64_t* __restrict__ ovf, const float* __restrict__ m6,
+                                                         uint32_t words, uint32_t cap, uint32_t* __restrict__ count,
+                                                         uint64_t* __restrict__ hash, uint32_t* __restrict__ keys) {
+    extern __shared__ uint32_t lds[];
+    __shared__ uint32_t list[SET_LIST_CAP];
+    __shared__ uint32_t list_n;
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t w = lane; w < words; w += 64) lds[w] = 0u;
+    if (lane == 0) list_n = 0u;
+    __syncthreads();
+    const uint32_t n = counters[C_OVERFLOW];
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const uint64_t e = ovf[i];
+        const SetDesc& sd = sets[(uint32_t)(e >> 32)];
+        const uint64_t out = sd.first + (uint32_t)e;
+        const uint32_t sw = (sd.cde.n_slots + 31u) / 32u;
+        collect_query(sd.cde, sd.item, load_mat(m6, out), out, sd.seed, sd.n_seed, sw, cap, lds, list, &list_n, count,
+                      hash, keys, lane);
     }
 }
+
+namespace jg {
+void collect_overflow_launch(const SetDesc* sets, const uint32_t* counters, const uint64_t* ovf, const float* m6,
+                             uint32_t words, uint32_t cap, uint32_t* count, uint64_t* hash, uint32_t* keys,
+                             uint32_t grid, hipStream_t stream) {
+    hipLaunchKernelGGL(k_collect_overflow, dim3(grid), dim3(64), (size_t)words * 4, stream, sets, counters, ovf, m6,
+                       words, cap, count, hash, keys);
+}
+}  // namespace jg
 
 // ------------------------------------------------------------------------------ ABI: polygons
 using namespace jg;
@@ -788,7 +870,7 @@ extern "C" jg_status jg_detect_batch(jg_cde* c, const jg_spolygon* item, const f
     s = item_upload(const_cast<jg_spolygon*>(item));
     if (s != JG_OK) return s;
     FilterView fv;
-    s = filter_view(c, filter, fv);
+    s = filter_view(c, filter, fv, (hipStream_t)stream);
     if (s != JG_OK) return s;
     ItemView iv = item->view;
     if (!(flags & JG_QUERY_SURROGATE)) { iv.n_poles = 0; iv.n_piers = 0; }
@@ -798,6 +880,7 @@ extern "C" jg_status jg_detect_batch(jg_cde* c, const jg_spolygon* item, const f
     hipLaunchKernelGGL(k_detect_plain, dim3((uint32_t)grid), dim3(block), 0, (hipStream_t)stream, c->view, iv, d_m6, n,
                        fv, flags, d_status);
     HIP_TRY(hipGetLastError());
+    if (fv.kind == 1) return filter_read_done(c, (hipStream_t)stream);
     return JG_OK;
 }
 

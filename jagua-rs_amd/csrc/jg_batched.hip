@@ -25,49 +25,13 @@
 
 #include "../../include/jagua_amd.h"
 #include "jg_internal.hpp"
+#include "jg_plan.hpp"
 
 using namespace jg;
 
 namespace {
 
-struct SetDesc {
-    CdeView cde;
-    ItemView item;
-    FilterView filter;
-    uint64_t first;
-    uint32_t flags;
-    uint32_t lanes;        // lanes per query in the polygon stage (power of two, 1..64)
-};
-
-struct Unit {              // 16 B
-    uint32_t set;
-    uint32_t q0;           // first query, relative to the set
-    uint32_t count;        // <= 64
-    uint32_t pad;
-};
-
-struct Group {             // 16 B: surviving queries q0 + i for the set bits i of mask
-    uint32_t set;
-    uint32_t q0;
-    uint64_t mask;
-};
-
-// Work is claimed from 8 streams, each with its own counters on their own 128-byte lines:
-// thousands of waves taking work with atomics on a single address queue behind each other
-// (measured: 1.81 ms against 1.30 ms per configuration-2 step for the survivor groups alone).
-//   unit stream x:  units x + 8 j; counter C_USTREAM0 + 32 x counts the units taken
-//   group stream x: the survivor groups of the units of unit stream x, in their own region of
-//                   the group array (base gbase[x]); C_GCOUNT0 + 32 x counts the groups
-//                   appended, C_STREAM0 + 32 x the groups taken
-// A wave starts on stream blockIdx.x mod 8 and moves to the next one as each runs dry.
-constexpr uint32_t N_STREAMS = 8, STREAM_STRIDE = 32;
-constexpr uint32_t C_STREAM0 = 0;
-constexpr uint32_t C_USTREAM0 = C_STREAM0 + STREAM_STRIDE * N_STREAMS;
-constexpr uint32_t C_GCOUNT0 = C_USTREAM0 + STREAM_STRIDE * N_STREAMS;
-constexpr uint32_t N_COUNTER_WORDS = C_GCOUNT0 + STREAM_STRIDE * N_STREAMS;
-
 // LDS of a workgroup (one wavefront): its work list
-constexpr uint32_t WAVE = 64;
 struct WaveLds {
     uint32_t nodes[WL_NCAP];
     uint32_t edges[WL_ECAP];
@@ -82,37 +46,7 @@ __device__ __forceinline__ WorkList wave_list(WaveLds& L) {
     return WorkList{L.nodes, L.edges, L.edge_ent, L.geom, L.ent_slot, L.hit, L.qidx};
 }
 
-__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
-
-// Lane 0's f32 value, bit for bit, in a scalar register (readfirstlane moves 32-bit integers).
-__device__ __forceinline__ float uniform_f32(float x) {
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
-}
-
-template <typename T>
-__device__ __forceinline__ T wave_bcast(T v) {
-    // lane 0's value in a scalar register: what is derived from it (work-unit records, the set
-    // descriptor and the engine's pointers) stays wave-uniform and is read with scalar loads
-    return (T)__builtin_amdgcn_readfirstlane((int)v);
-}
-
 }  // namespace
-
-// The next item of n from 8 interleaved streams (item x + 8 j in stream x, counters from word
-// base on): the wave's stream x first, then the following ones as they drain; n when all are.
-__device__ __forceinline__ uint32_t claim_next(uint32_t* counters, uint32_t base, uint32_t& x, uint32_t& tried,
-                                               uint32_t n, uint32_t lane) {
-    while (tried < N_STREAMS) {
-        uint32_t j = 0;
-        if (lane == 0) j = atomicAdd(&counters[base + STREAM_STRIDE * x], 1u);
-        j = __builtin_amdgcn_readfirstlane(j);
-        const uint64_t i = (uint64_t)x + (uint64_t)N_STREAMS * j;
-        if (i < n) return (uint32_t)i;
-        x = (x + 1u) & (N_STREAMS - 1u);
-        tried++;
-    }
-    return n;
-}
 
 __global__ void k_reset_counters(uint32_t* counters) {
     for (uint32_t i = threadIdx.x; i < N_COUNTER_WORDS; i += blockDim.x) counters[i] = 0u;
@@ -213,43 +147,6 @@ __global__ __launch_bounds__(WAVE) void k_surrogate(const SetDesc* __restrict__ 
         }
         wave_sync();
     }
-}
-
-// Sum of v over the G lanes of this lane's aligned group (G a power of two <= 64).
-__device__ __forceinline__ int group_sum_i32(int v, uint32_t G) {
-    for (uint32_t o = G >> 1; o > 0; o >>= 1) v += __shfl_xor(v, (int)o, 64);
-    return v;
-}
-
-// pip_hazard / pip_query with the G lanes of a group splitting the polygon's edges (lane sub
-// takes edges sub, sub + G, ...).  Each edge's crossing count is the sequential loop's, and
-// their sum is an integer, so the parity is the same.  The group's lanes call them together
-// with the same arguments.
-__device__ __forceinline__ bool pip_hazard_group(const CdeView& c, uint32_t slot, float px, float py, uint32_t sub,
-                                                 uint32_t G) {
-    const float4 hb = c.haz_bbox[slot];
-    if (!box_has_point(hb, px, py)) return false;
-    const float ray_x = hb.z + (hb.z - hb.x);
-    const uint2 vr = c.haz_vr[slot];
-    const float2* v = c.haz_verts + vr.x;
-    int cnt = 0;
-    for (uint32_t i = sub; i < vr.y; i += G) {
-        const float2 a = v[i], b = v[i + 1 == vr.y ? 0 : i + 1];
-        cnt += pip_edge(px, py, ray_x, a.x, a.y, b.x, b.y);
-    }
-    return (group_sum_i32(cnt, G) & 1) == 1;
-}
-__device__ __forceinline__ bool pip_query_group(const ItemView& it, const Mat& m, float4 qbb, float px, float py,
-                                                uint32_t sub, uint32_t G) {
-    if (!box_has_point(qbb, px, py)) return false;
-    const float ray_x = qbb.z + (qbb.z - qbb.x);
-    int cnt = 0;
-    for (uint32_t i = sub; i < it.n_v; i += G) {
-        const uint32_t k = i + 1 == it.n_v ? 0 : i + 1;
-        const float2 a = xform(m, it.v[i].x, it.v[i].y), b = xform(m, it.v[k].x, it.v[k].y);
-        cnt += pip_edge(px, py, ray_x, a.x, a.y, b.x, b.y);
-    }
-    return (group_sum_i32(cnt, G) & 1) == 1;
 }
 
 // detect_containment_collision (cd_engine.rs:231-255) over the candidates of one query, the
@@ -584,20 +481,194 @@ __global__ __launch_bounds__(WAVE, 4) void k_fused_filtered(const SetDesc* __res
 }
 
 // ---------------------------------------------------------------------------------------
+namespace {
+
+
+// JG_FILTER_KEYS masks of the sets, one after another: set i's at words[off[i]..].
+void filter_masks(const std::vector<jg_query_set>& sets, const std::vector<jg_hazard_filter>& filters,
+                  const std::vector<std::vector<jg_hazkey>>& keys, std::vector<uint32_t>& words,
+                  std::vector<size_t>& off) {
+    words.clear();
+    off.assign(sets.size(), 0);
+    for (size_t i = 0; i < sets.size(); i++) {
+        if (filters[i].kind != JG_FILTER_KEYS) continue;
+        const jg_cde* c = sets[i].cde;
+        const uint32_t n = (uint32_t)((c->map.slots.size() + 31) / 32);
+        off[i] = words.size();
+        words.resize(words.size() + n, 0u);
+        for (const jg_hazkey& k : keys[i])
+            if (c->map.valid(k)) words[off[i] + (k.idx >> 5)] |= 1u << (k.idx & 31u);
+    }
+}
+
+FilterView set_filter(const jg_query_set& s, const jg_hazard_filter& f, const uint32_t* mask) {
+    FilterView v{nullptr, 0, 0, 0, 0};
+    if (f.kind == JG_FILTER_HASHED) {
+        v.kind = 2;
+        v.seed = f.seed;
+        v.threshold = f.threshold;
+    } else if (f.kind == JG_FILTER_KEYS) {
+        v.kind = 1;
+        v.mask = mask;
+        v.n_words = (uint32_t)((s.cde->map.slots.size() + 31) / 32);
+    }
+    return v;
+}
+
+// The host bytes of the table of `sets` under schedule `cfg`, given each set's item view and
+// filter view (device addresses), and its layout.
+// seeds[i] (device address, count): the slots set i's collectors start with (may be empty).
+jg_status table_image(const std::vector<jg_query_set>& sets, const std::vector<ItemView>& iv,
+                      const std::vector<FilterView>& fv, const std::vector<std::pair<const uint32_t*, uint32_t>>& seeds,
+                      const jg_plan_config& cfg, uint32_t n_cus, std::vector<uint8_t>& host, TableLayout& L) {
+    for (auto& s : sets)
+        if (s.cde->view.n_nodes >= WL_MAX_NODES)
+            return fail(JG_ERR_CAPACITY, "quadtree of more than 2^24 nodes in a batched plan");
+    std::vector<SetDesc> sd(sets.size());
+    bool any_filter = false, any_pole = false, any_pier = false;
+    L.max_slot_words = 1;
+    L.n_queries = 0;
+    for (size_t i = 0; i < sets.size(); i++) {
+        const jg_query_set& s = sets[i];
+        std::memset(&sd[i], 0, sizeof(SetDesc));
+        L.max_slot_words = std::max<uint32_t>(L.max_slot_words, (s.cde->view.n_slots + 31u) / 32u);
+        L.n_queries += s.count;
+        if (i < seeds.size()) {
+            sd[i].seed = seeds[i].first;
+            sd[i].n_seed = seeds[i].second;
+        }
+        sd[i].cde = s.cde->view;
+        sd[i].item = iv[i];
+        sd[i].flags = cfg.flags ? cfg.flags : (JG_QUERY_SURROGATE | JG_QUERY_POLY);
+        if (!(sd[i].flags & JG_QUERY_SURROGATE)) { sd[i].item.n_poles = 0; sd[i].item.n_piers = 0; }
+        sd[i].first = s.first;
+        uint32_t lanes = cfg.lanes_per_query;
+        if (lanes == 0) {
+            // the vertex count rounded up to a power of two, 8..32: items above 32 vertices run
+            // two queries per pass, which measured faster than one query on 64 lanes
+            lanes = 8;
+            while (lanes < s.item->poly.n() && lanes < 32u) lanes <<= 1;
+        }
+        uint32_t pow2 = 1;
+        while (pow2 < lanes && pow2 < 64u) pow2 <<= 1;
+        sd[i].lanes = pow2;
+        sd[i].filter = fv[i];
+        any_filter = any_filter || sd[i].filter.kind != 0;
+        any_pole = any_pole || sd[i].item.n_poles > 0;
+        any_pier = any_pier || sd[i].item.n_piers > 0;
+    }
+    L.filtered = any_filter;
+    L.kind = any_pier ? (any_pole ? KIND_MIXED : KIND_EDGE) : KIND_CIRCLE;
+    L.stages = cfg.stages == 1u ? 1u : 2u;
+    // work units: sets in decreasing item size, `chunk` queries each (16, 32 or 64; 64 packs
+    // 128 pole entities into one surrogate traversal; smaller units shorten the fused kernel's
+    // tail)
+    const uint32_t uq = cfg.chunk == 0 ? 64u : cfg.chunk <= 16u ? 16u : cfg.chunk <= 32u ? 32u : 64u;
+    std::vector<size_t> order(sets.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return sets[a].item->poly.n() > sets[b].item->poly.n(); });
+    std::vector<Unit> units;
+    for (size_t oi : order) {
+        const jg_query_set& s = sets[oi];
+        for (uint64_t q = 0; q < s.count; q += uq)
+            units.push_back(Unit{(uint32_t)oi, (uint32_t)q, (uint32_t)std::min<uint64_t>(uq, s.count - q), 0u});
+    }
+    if (units.size() >= 0xffffffffull) return fail(JG_ERR_CAPACITY, "too many work units in a batched plan");
+    L.n_units = (uint32_t)units.size();
+    L.o_units = (sizeof(SetDesc) * sd.size() + 255) & ~(size_t)255;
+    L.o_ctr = (L.o_units + sizeof(Unit) * units.size() + 255) & ~(size_t)255;
+    L.o_gbase = L.o_ctr + 4u * N_COUNTER_WORDS;
+    // one group per polygon-stage pass at most; unit u's groups go to the region of stream u mod 8
+    uint64_t stream_groups[N_STREAMS] = {};
+    for (size_t u = 0; u < units.size(); u++) {
+        const uint32_t per = 64u / sd[units[u].set].lanes;
+        stream_groups[u & (N_STREAMS - 1u)] += (units[u].count + per - 1u) / per;
+    }
+    uint32_t gbase[N_STREAMS];
+    uint64_t max_groups = 0;
+    for (uint32_t x = 0; x < N_STREAMS; x++) {
+        gbase[x] = (uint32_t)max_groups;
+        max_groups += stream_groups[x];
+        if (max_groups >= 0xffffffffull) return fail(JG_ERR_CAPACITY, "too many survivor groups in a batched plan");
+    }
+    L.o_grp = (L.o_gbase + sizeof(gbase) + 255) & ~(size_t)255;
+    L.total = L.o_grp + sizeof(Group) * (max_groups + 1);
+    host.assign(L.o_grp, 0);                 // the group array is written by the kernels
+    std::memcpy(host.data(), sd.data(), sizeof(SetDesc) * sd.size());
+    std::memcpy(host.data() + L.o_units, units.data(), sizeof(Unit) * units.size());
+    std::memcpy(host.data() + L.o_gbase, gbase, sizeof(gbase));
+    // one wavefront per workgroup, each taking one work unit (or survivor group) at a time
+    const uint32_t want = cfg.workgroups ? cfg.workgroups : n_cus * 16u;
+    L.grid_s = std::max<uint32_t>(1u, std::min<uint32_t>(want, L.n_units));
+    L.grid_p = std::max<uint32_t>(1u, (uint32_t)std::min<uint64_t>(want, max_groups));
+    return JG_OK;
+}
+
+// The launch sequence over a table at device address `base`: the counter reset (a kernel, so
+// that the sequence can be captured in a graph), then the two stages or the fused kernel.
+jg_status table_launch(const TableLayout& L, uint8_t* base, const float* d_m6, uint8_t* d_status,
+                       hipStream_t stream) {
+    uint32_t* counters = (uint32_t*)(base + L.o_ctr);
+    Group* groups = (Group*)(base + L.o_grp);
+    const uint32_t* gbase = (const uint32_t*)(base + L.o_gbase);
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream, counters);
+    const SetDesc* sets = (const SetDesc*)base;
+    const Unit* units = (const Unit*)(base + L.o_units);
+    const dim3 gs(L.grid_s), gp(L.grid_p), blk(WAVE);
+    if (L.stages == 1u) {
+#define JG_FUSED(KER, K) hipLaunchKernelGGL((KER<K>), gs, blk, 0, stream, sets, units, L.n_units, counters, d_m6, d_status)
+        if (L.filtered) {
+            if (L.kind == KIND_CIRCLE) JG_FUSED(k_fused_filtered, KIND_CIRCLE);
+            else if (L.kind == KIND_EDGE) JG_FUSED(k_fused_filtered, KIND_EDGE);
+            else JG_FUSED(k_fused_filtered, KIND_MIXED);
+        } else {
+            if (L.kind == KIND_CIRCLE) JG_FUSED(k_fused, KIND_CIRCLE);
+            else if (L.kind == KIND_EDGE) JG_FUSED(k_fused, KIND_EDGE);
+            else JG_FUSED(k_fused, KIND_MIXED);
+        }
+#undef JG_FUSED
+        return hip_check(hipGetLastError(), "batched query launch");
+    }
+#define JG_SURR(F, K) hipLaunchKernelGGL((k_surrogate<F, K>), gs, blk, 0, stream, sets, units, L.n_units, counters, \
+                                         groups, gbase, d_m6, d_status)
+    if (L.filtered) {
+        if (L.kind == KIND_CIRCLE) JG_SURR(true, KIND_CIRCLE);
+        else if (L.kind == KIND_EDGE) JG_SURR(true, KIND_EDGE);
+        else JG_SURR(true, KIND_MIXED);
+        hipLaunchKernelGGL(k_poly_filtered, gp, blk, 0, stream, sets, counters, (const Group*)groups, gbase, d_m6,
+                           d_status);
+    } else {
+        if (L.kind == KIND_CIRCLE) JG_SURR(false, KIND_CIRCLE);
+        else if (L.kind == KIND_EDGE) JG_SURR(false, KIND_EDGE);
+        else JG_SURR(false, KIND_MIXED);
+        hipLaunchKernelGGL(k_poly, gp, blk, 0, stream, sets, counters, (const Group*)groups, gbase, d_m6, d_status);
+    }
+#undef JG_SURR
+    return hip_check(hipGetLastError(), "batched query launch");
+}
+
+uint32_t device_cus() {
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    return (uint32_t)cus;
+}
+
+}  // namespace
+
 struct jg_plan {
     std::vector<jg_query_set> sets;
     std::vector<jg_hazard_filter> filters;
     std::vector<std::vector<jg_hazkey>> filter_keys;
     jg_plan_config cfg{};
     DevBlob table;                  // SetDesc[], Unit[], counters, survivor groups
-    DevBlob masks;                  // JG_FILTER_KEYS masks of every set
+    DevBlob masks;                  // JG_FILTER_KEYS masks of every set, then their slots ascending
+    DevBlob ovf;                    // collection: the queries handed to the per-query collector
+    TableLayout L;
+    // engine image generations and item upload generations the table was built from
     std::vector<uint64_t> generations;
-    uint32_t n_units = 0;
-    size_t off_units = 0, off_counter = 0, off_groups = 0, off_gbase = 0;
-    uint32_t grid_s = 0, grid_p = 0;
     uint32_t n_cus = 0;
-    bool filtered = false;          // some set has a filter (kernels with the filter test)
-    int kind = KIND_CIRCLE;         // entity kinds of the surrogate stage over all sets
     // graph cache
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -620,119 +691,58 @@ jg_status plan_build_table(jg_plan* p) {
         st = item_upload(const_cast<jg_spolygon*>(s.item));
         if (st != JG_OK) return st;
     }
-    for (auto& s : p->sets)
-        if (s.cde->view.n_nodes >= WL_MAX_NODES)
-            return fail(JG_ERR_CAPACITY, "quadtree of more than 2^24 nodes in a batched plan");
+    // the table holds engine views and item views: it is rebuilt when an engine image or an
+    // item record (a regenerated surrogate, say) was uploaded anew
     std::vector<uint64_t> gens;
-    for (auto& s : p->sets) gens.push_back(s.cde->generation);
+    for (auto& s : p->sets) {
+        gens.push_back(s.cde->generation);
+        gens.push_back(s.item->upload_gen);
+    }
     if (gens == p->generations && p->table.ptr) return JG_OK;
     if (p->table.ptr) {             // launches in flight on any stream may still read the table
         jg_status e = hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
         if (e != JG_OK) return e;
     }
-
-    // filter masks
     std::vector<uint32_t> mask_words;
-    std::vector<size_t> mask_off(p->sets.size(), 0);
+    std::vector<size_t> mask_off;
+    filter_masks(p->sets, p->filters, p->filter_keys, mask_words, mask_off);
+    // the key filters' slots, ascending: the seeds of the sets' collectors
+    std::vector<size_t> seed_off(p->sets.size(), 0);
+    std::vector<uint32_t> seed_n(p->sets.size(), 0);
     for (size_t i = 0; i < p->sets.size(); i++) {
-        const jg_hazard_filter& f = p->filters[i];
-        if (f.kind != JG_FILTER_KEYS) continue;
-        jg_cde* c = p->sets[i].cde;
-        const uint32_t words = (uint32_t)((c->map.slots.size() + 31) / 32);
-        mask_off[i] = mask_words.size();
-        mask_words.resize(mask_words.size() + words, 0u);
-        for (const jg_hazkey& k : p->filter_keys[i])
-            if (c->map.valid(k)) mask_words[mask_off[i] + (k.idx >> 5)] |= 1u << (k.idx & 31u);
+        if (p->filters[i].kind != JG_FILTER_KEYS) continue;
+        const uint32_t words = (uint32_t)((p->sets[i].cde->map.slots.size() + 31) / 32);
+        seed_off[i] = mask_words.size();
+        std::vector<uint32_t> sl;
+        for (uint32_t w = 0; w < words; w++)
+            for (uint32_t b = 0; b < 32; b++)
+                if ((mask_words[mask_off[i] + w] >> b) & 1u) sl.push_back(w * 32u + b);
+        mask_words.insert(mask_words.end(), sl.begin(), sl.end());
+        seed_n[i] = (uint32_t)sl.size();
     }
     if (!mask_words.empty()) {
         jg_status st = p->masks.reserve(mask_words.size() * 4);
         if (st != JG_OK) return st;
-        jg_status e = hip_check(hipMemcpy(p->masks.ptr, mask_words.data(), mask_words.size() * 4, hipMemcpyHostToDevice),
-                                "hipMemcpy masks");
-        if (e != JG_OK) return e;
+        st = hip_check(hipMemcpy(p->masks.ptr, mask_words.data(), mask_words.size() * 4, hipMemcpyHostToDevice),
+                       "hipMemcpy masks");
+        if (st != JG_OK) return st;
     }
-    std::vector<SetDesc> sd(p->sets.size());
-    bool any_filter = false, any_pole = false, any_pier = false;
+    std::vector<ItemView> iv;
+    std::vector<FilterView> fv;
+    std::vector<std::pair<const uint32_t*, uint32_t>> seeds;
     for (size_t i = 0; i < p->sets.size(); i++) {
-        const jg_query_set& s = p->sets[i];
-        sd[i].cde = s.cde->view;
-        sd[i].item = s.item->view;
-        sd[i].flags = p->cfg.flags ? p->cfg.flags : (JG_QUERY_SURROGATE | JG_QUERY_POLY);
-        if (!(sd[i].flags & JG_QUERY_SURROGATE)) { sd[i].item.n_poles = 0; sd[i].item.n_piers = 0; }
-        sd[i].first = s.first;
-        uint32_t lanes = p->cfg.lanes_per_query;
-        if (lanes == 0) {
-            // the vertex count rounded up to a power of two, 8..32: items above 32 vertices run
-            // two queries per pass, which measured faster than one query on 64 lanes
-            lanes = 8;
-            while (lanes < s.item->poly.n() && lanes < 32u) lanes <<= 1;
-        }
-        uint32_t pow2 = 1;
-        while (pow2 < lanes && pow2 < 64u) pow2 <<= 1;
-        sd[i].lanes = pow2;
-        const jg_hazard_filter& f = p->filters[i];
-        sd[i].filter = FilterView{nullptr, 0, 0, 0, 0};
-        if (f.kind == JG_FILTER_HASHED) {
-            sd[i].filter.kind = 2;
-            sd[i].filter.seed = f.seed;
-            sd[i].filter.threshold = f.threshold;
-        } else if (f.kind == JG_FILTER_KEYS) {
-            sd[i].filter.kind = 1;
-            sd[i].filter.mask = (const uint32_t*)p->masks.ptr + mask_off[i];
-            sd[i].filter.n_words = (uint32_t)((s.cde->map.slots.size() + 31) / 32);
-        }
-        any_filter = any_filter || sd[i].filter.kind != 0;
-        any_pole = any_pole || sd[i].item.n_poles > 0;
-        any_pier = any_pier || sd[i].item.n_piers > 0;
+        iv.push_back(p->sets[i].item->view);
+        fv.push_back(set_filter(p->sets[i], p->filters[i], (const uint32_t*)p->masks.ptr + mask_off[i]));
+        seeds.emplace_back(seed_n[i] ? (const uint32_t*)p->masks.ptr + seed_off[i] : nullptr, seed_n[i]);
     }
-    p->filtered = any_filter;
-    p->kind = any_pier ? (any_pole ? KIND_MIXED : KIND_EDGE) : KIND_CIRCLE;
-    // work units: sets in decreasing item size, `chunk` queries each (16, 32 or 64; 64 packs
-    // 128 pole entities into one surrogate traversal; smaller units shorten the fused kernel's
-    // tail)
-    const uint32_t uq = p->cfg.chunk == 0 ? 64u : p->cfg.chunk <= 16u ? 16u : p->cfg.chunk <= 32u ? 32u : 64u;
-    std::vector<size_t> order(p->sets.size());
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
-        return p->sets[a].item->poly.n() > p->sets[b].item->poly.n();
-    });
-    std::vector<Unit> units;
-    for (size_t oi : order) {
-        const jg_query_set& s = p->sets[oi];
-        for (uint64_t q = 0; q < s.count; q += uq)
-            units.push_back(Unit{(uint32_t)oi, (uint32_t)q, (uint32_t)std::min<uint64_t>(uq, s.count - q), 0u});
-    }
-    p->n_units = (uint32_t)units.size();
-    const size_t o_units = (sizeof(SetDesc) * sd.size() + 255) & ~(size_t)255;
-    const size_t o_ctr = (o_units + sizeof(Unit) * units.size() + 255) & ~(size_t)255;
-    const size_t o_gbase = o_ctr + 4u * N_COUNTER_WORDS;
-    // one group per polygon-stage pass at most; unit u's groups go to the region of stream u mod 8
-    uint64_t stream_groups[N_STREAMS] = {};
-    for (size_t u = 0; u < units.size(); u++) {
-        const uint32_t per = 64u / sd[units[u].set].lanes;
-        stream_groups[u & (N_STREAMS - 1u)] += (units[u].count + per - 1u) / per;
-    }
-    uint32_t gbase[N_STREAMS];
-    size_t max_groups = 0;
-    for (uint32_t x = 0; x < N_STREAMS; x++) {
-        gbase[x] = (uint32_t)max_groups;
-        max_groups += stream_groups[x];
-    }
-    if (max_groups >= 0xffffffffull) return fail(JG_ERR_CAPACITY, "too many survivor groups in a batched plan");
-    const size_t o_grp = (o_gbase + sizeof(gbase) + 255) & ~(size_t)255;
-    const size_t total = o_grp + sizeof(Group) * (max_groups + 1);
-    std::vector<uint8_t> host(total, 0);
-    std::memcpy(host.data(), sd.data(), sizeof(SetDesc) * sd.size());
-    std::memcpy(host.data() + o_units, units.data(), sizeof(Unit) * units.size());
-    std::memcpy(host.data() + o_gbase, gbase, sizeof(gbase));
-    jg_status st = p->table.reserve(total);
+    if (!p->n_cus) p->n_cus = device_cus();
+    std::vector<uint8_t> host;
+    jg_status st = table_image(p->sets, iv, fv, seeds, p->cfg, p->n_cus, host, p->L);
     if (st != JG_OK) return st;
-    st = hip_check(hipMemcpy(p->table.ptr, host.data(), total, hipMemcpyHostToDevice), "hipMemcpy plan");
+    st = p->table.reserve(p->L.total);
     if (st != JG_OK) return st;
-    p->off_units = o_units;
-    p->off_counter = o_ctr;
-    p->off_groups = o_grp;
-    p->off_gbase = o_gbase;
+    st = hip_check(hipMemcpy(p->table.ptr, host.data(), host.size(), hipMemcpyHostToDevice), "hipMemcpy plan");
+    if (st != JG_OK) return st;
     p->generations = gens;
     if (p->gexec) {                 // a captured graph refers to the old table
         (void)hipGraphExecDestroy(p->gexec);
@@ -740,56 +750,11 @@ jg_status plan_build_table(jg_plan* p) {
         p->gexec = nullptr;
         p->graph = nullptr;
     }
-    int dev = 0, cus = 256;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    p->n_cus = (uint32_t)cus;
-    // one wavefront per workgroup, each taking one work unit (or survivor group) at a time
-    const uint32_t want = p->cfg.workgroups ? p->cfg.workgroups : (uint32_t)cus * 16u;
-    p->grid_s = std::max<uint32_t>(1u, std::min<uint32_t>(want, p->n_units));
-    p->grid_p = std::max<uint32_t>(1u, (uint32_t)std::min<size_t>(want, max_groups));
     return JG_OK;
 }
 
 jg_status plan_launch(jg_plan* p, const float* d_m6, uint8_t* d_status, hipStream_t stream) {
-    uint8_t* base = (uint8_t*)p->table.ptr;
-    uint32_t* counters = (uint32_t*)(base + p->off_counter);
-    Group* groups = (Group*)(base + p->off_groups);
-    const uint32_t* gbase = (const uint32_t*)(base + p->off_gbase);
-    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream, counters);
-    const SetDesc* sets = (const SetDesc*)base;
-    const Unit* units = (const Unit*)(base + p->off_units);
-    const dim3 gs(p->grid_s), gp(p->grid_p), blk(WAVE);
-    if (p->cfg.stages == 1u) {
-#define JG_FUSED(KER, K) hipLaunchKernelGGL((KER<K>), gs, blk, 0, stream, sets, units, p->n_units, counters, d_m6, d_status)
-        if (p->filtered) {
-            if (p->kind == KIND_CIRCLE) JG_FUSED(k_fused_filtered, KIND_CIRCLE);
-            else if (p->kind == KIND_EDGE) JG_FUSED(k_fused_filtered, KIND_EDGE);
-            else JG_FUSED(k_fused_filtered, KIND_MIXED);
-        } else {
-            if (p->kind == KIND_CIRCLE) JG_FUSED(k_fused, KIND_CIRCLE);
-            else if (p->kind == KIND_EDGE) JG_FUSED(k_fused, KIND_EDGE);
-            else JG_FUSED(k_fused, KIND_MIXED);
-        }
-#undef JG_FUSED
-        return hip_check(hipGetLastError(), "batched query launch");
-    }
-#define JG_SURR(F, K) hipLaunchKernelGGL((k_surrogate<F, K>), gs, blk, 0, stream, sets, units, p->n_units, counters, \
-                                         groups, gbase, d_m6, d_status)
-    if (p->filtered) {
-        if (p->kind == KIND_CIRCLE) JG_SURR(true, KIND_CIRCLE);
-        else if (p->kind == KIND_EDGE) JG_SURR(true, KIND_EDGE);
-        else JG_SURR(true, KIND_MIXED);
-        hipLaunchKernelGGL(k_poly_filtered, gp, blk, 0, stream, sets, counters, (const Group*)groups, gbase, d_m6,
-                           d_status);
-    } else {
-        if (p->kind == KIND_CIRCLE) JG_SURR(false, KIND_CIRCLE);
-        else if (p->kind == KIND_EDGE) JG_SURR(false, KIND_EDGE);
-        else JG_SURR(false, KIND_MIXED);
-        hipLaunchKernelGGL(k_poly, gp, blk, 0, stream, sets, counters, (const Group*)groups, gbase, d_m6, d_status);
-    }
-#undef JG_SURR
-    return hip_check(hipGetLastError(), "batched query launch");
+    return table_launch(p->L, (uint8_t*)p->table.ptr, d_m6, d_status, stream);
 }
 
 }  // namespace
@@ -837,7 +802,7 @@ extern "C" void jg_plan_free(jg_plan* p) { delete p; }
 
 extern "C" jg_status jg_plan_detect(jg_plan* p, const float* d_m6, uint8_t* d_status, void* stream) {
     if (!p) return fail(JG_ERR_INVALID_ARG, "null plan");
-    if (p->n_units == 0) return JG_OK;
+    if (p->L.n_units == 0) return JG_OK;
     if (!d_m6 || !d_status) return fail(JG_ERR_INVALID_ARG, "null device buffer");
     jg_status st = plan_build_table(p);
     if (st != JG_OK) return st;
@@ -846,7 +811,7 @@ extern "C" jg_status jg_plan_detect(jg_plan* p, const float* d_m6, uint8_t* d_st
 
 extern "C" jg_status jg_plan_detect_graph(jg_plan* p, const float* d_m6, uint8_t* d_status, uint32_t replays, void* stream) {
     if (!p) return fail(JG_ERR_INVALID_ARG, "null plan");
-    if (p->n_units == 0 || replays == 0) return JG_OK;
+    if (p->L.n_units == 0 || replays == 0) return JG_OK;
     if (!d_m6 || !d_status) return fail(JG_ERR_INVALID_ARG, "null device buffer");
     jg_status st = plan_build_table(p);
     if (st != JG_OK) return st;
@@ -885,6 +850,26 @@ extern "C" jg_status jg_plan_detect_graph(jg_plan* p, const float* d_m6, uint8_t
         if (st != JG_OK) return st;
     }
     return JG_OK;
+}
+
+extern "C" jg_status jg_plan_collect(jg_plan* p, const float* d_m6, uint32_t capacity, uint32_t* d_count,
+                                     uint64_t* d_hash, uint32_t* d_keys, void* stream) {
+    if (!p) return fail(JG_ERR_INVALID_ARG, "null plan");
+    for (const jg_hazard_filter& f : p->filters)
+        if (f.kind == JG_FILTER_HASHED)
+            return fail(JG_ERR_INVALID_ARG, "hazard collection takes no hashed filter (a collector is seeded with keys)");
+    for (const jg_query_set& s : p->sets)
+        if (!s.item->poly.sur.present || s.item->poly.sur.cfg.n_ff_poles > s.item->poly.sur.poles.size())
+            return fail(JG_ERR_SURROGATE, "collection needs every item's surrogate (ff poles)");
+    if (p->L.n_units == 0) return JG_OK;
+    if (!d_m6 || !d_count || !d_hash) return fail(JG_ERR_INVALID_ARG, "null device buffer");
+    jg_status st = plan_build_table(p);
+    if (st != JG_OK) return st;
+    if ((size_t)p->L.max_slot_words * 4 > 60000) return fail(JG_ERR_CAPACITY, "too many hazard slots for the LDS collector");
+    st = p->ovf.reserve(sizeof(uint64_t) * p->L.n_queries);
+    if (st != JG_OK) return st;
+    return collect_launch(p->L, (uint8_t*)p->table.ptr, d_m6, capacity, d_count, d_hash, d_keys, (uint64_t*)p->ovf.ptr,
+                          p->n_cus, (hipStream_t)stream);
 }
 
 extern "C" jg_status jg_plan_get_config(const jg_plan* p, jg_plan_config* out) {
@@ -926,7 +911,7 @@ extern "C" jg_status jg_plan_autotune(jg_plan* p, const float* d_m6, uint8_t* d_
     if (!p) return fail(JG_ERR_INVALID_ARG, "null plan");
     if (!d_m6 || !d_status) return fail(JG_ERR_INVALID_ARG, "null device buffer");
     if (reps == 0) reps = 3;
-    if (p->n_units == 0) {
+    if (p->L.n_units == 0) {
         if (best) *best = p->cfg;
         if (best_ms) *best_ms = 0.0f;
         return JG_OK;

@@ -1,0 +1,585 @@
+// Batched hazard collection (jg_plan_collect): the hazard set of every query of a plan, as
+// ci_bench composes it (lbf/benches/ci_bench.rs:56-69): collect_surrogate_collisions
+// (cd_engine.rs:296-310) then collect_poly_collisions (cd_engine.rs:261-289), in persistent
+// one-wave workgroups that claim the plan's work units.
+//
+// A wave collects CP queries of a unit at a time.  All their entities (ff poles and piers from
+// the root, query edges from each query's virtual root) share one wave traversal in the manner
+// of the detection's (node visits and edge tests queued in LDS, 64 per step), with
+// QTNode::collect_collisions (qt_node.rs:151-191) as the visit: a node whose active edges number
+// at most cd_threshold, or a leaf, inserts its Entire hazards and queues edge tests for its
+// Partial hazards; other nodes queue their colliding quadrants.  Each query's collector is a
+// short list of slots in LDS with union semantics (collector.rs:44-67).
+//
+// Subtree skip.  A set is a union, so a visit that cannot add a slot can be left out.  Below a
+// node, an entity collects a hazard only where it meets it: through an edge test, which fails
+// unless the two edges' bounding boxes overlap (edge.rs:187-196), or at an Entire node of the
+// hazard that the entity reaches (inside the hazard).  Every hazard present below a node is an
+// entry of the node (constrict only narrows).  Hazard boxes are widened by 2^-14 of their
+// coordinates' magnitude, far above the few ulps by which an Entire node or a quadrant flag may
+// stray past a box in f32.  The exterior hazard's Entire nodes lie outside the container, so it
+// can be ruled out only for an entity inside a box that is inside the container (the
+// container's bbox when it is an axis-aligned rectangle), shrunk by the same margin.
+//  - query edges: everything they can collect is an entry of the query's virtual root whose
+//    widened box meets the query's bbox.  Those candidates (up to NC) are listed per query when
+//    the pass starts, with each edge's mask of candidates its box meets; a visit is skipped
+//    once every candidate of its edge's mask is in the set.  The skip reads only LDS.
+//  - ff poles and piers descend from the root: a visit to a node of at most PRUNE_MAX entries
+//    is skipped when no entry outside the set may be collected by the entity's box.
+// The candidates also hold every hazard the containment loop can add (it needs a bbox
+// relation, or the exterior), so that loop runs over them.  On the configuration-2 layout
+// the skip takes the oracle's 525 node visits per query to about 150.
+//
+// A query whose set outgrows the LDS list (SET_CAP slots) is handed to the per-query collector
+// (k_collect_overflow), which keeps a bit per slot.  A query with more than NC candidates skips
+// no visit of its edges and scans all its virtual root's entries for containment.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "../../include/jagua_amd.h"
+#include "jg_internal.hpp"
+#include "jg_plan.hpp"
+
+using namespace jg;
+
+namespace {
+
+constexpr uint32_t CP = 16;              // queries per collection pass
+constexpr uint32_t CG = WAVE / CP;       // lanes per query in the set-up, containment and result phases
+constexpr uint32_t SET_CAP = 16;         // slots per query collector in LDS
+constexpr uint32_t NC = 16;              // candidates per query (bits of an edge's mask)
+constexpr uint32_t CE_CAP = 256;         // edge-test queue entries
+constexpr uint32_t CE_LIMIT = 256;       // edge tests one visit pushes: fewer (also below the scan's 16 bits)
+static_assert(64u * (CE_LIMIT - 1u) < 65536u, "a wave's edge pushes must fit 16 bits");
+constexpr uint32_t PRUNE_MAX = 8;        // surrogate visits: nodes with at most this many entries are checked
+constexpr uint32_t SURR_MASK = 0u;       // emask of a pole or pier: the node-entry check applies
+constexpr uint32_t C_EXT = 0x80000000u;  // candidate record flags: exterior hazard, Entire in the virtual root
+constexpr uint32_t C_ENTIRE = 0x40000000u;
+constexpr uint32_t C_SLOT = 0x3fffffffu;
+
+struct CollectLds {
+    uint32_t nodes[WL_NCAP];             // entity << 24 | node
+    uint32_t edges[CE_CAP];              // hazard edge index
+    uint32_t eslot[CE_CAP];              // its hazard's slot
+    float4 geom[WL_ENT];                 // circle (cx, cy, r, -) or edge (x1 y1 x2 y2)
+    uint32_t emask[WL_ENT];              // query edge: its candidates; pole or pier: SURR_MASK
+    uint32_t set[CP * SET_CAP];          // per query: slot + 1, filled from the front, 0 = free
+    uint32_t cand[CP * NC];              // per query: candidate slot | C_EXT | C_ENTIRE
+    float4 cbox[CP * NC];                // its widened box (the exterior: the inner box, shrunk)
+    uint32_t ncand[CP];                  // candidates found (more than NC: none are used)
+    uint32_t cdone[CP];                  // bit i: candidate i is in the set
+    uint32_t ovf[CP];                    // per query: its set outgrew SET_CAP
+    int32_t vroot[CP];
+    uint8_t edge_ent[CE_CAP];
+    uint8_t ent_slot[WL_ENT];            // query (0..CP-1) | WL_CIRCLE
+};
+
+__device__ __forceinline__ bool set_has(const CollectLds& L, uint32_t q, uint32_t slot) {
+    const uint32_t key = slot + 1u;
+    const uint32_t* t = L.set + q * SET_CAP;
+    for (uint32_t i = 0; i < SET_CAP; i++) {
+        const uint32_t v = __atomic_load_n(&t[i], __ATOMIC_RELAXED);
+        if (v == key) return true;
+        if (v == 0u) return false;
+    }
+    return false;
+}
+
+// Inserts at the first free place, so a set stays a prefix; concurrent inserts of one slot keep
+// one copy (the compare-and-swap sees the other's key).  A new slot that is one of the query's
+// candidates marks it done.
+__device__ __forceinline__ void set_put(CollectLds& L, uint32_t q, uint32_t slot) {
+    const uint32_t key = slot + 1u;
+    uint32_t* t = L.set + q * SET_CAP;
+    for (uint32_t i = 0; i < SET_CAP; i++) {
+        const uint32_t old = atomicCAS(&t[i], 0u, key);
+        if (old == key) return;
+        if (old == 0u) {
+            const uint32_t nc = L.ncand[q];
+            if (nc <= NC)
+                for (uint32_t k = 0; k < nc; k++)
+                    if ((L.cand[q * NC + k] & C_SLOT) == slot) atomicOr(&L.cdone[q], 1u << k);
+            return;
+        }
+    }
+    atomicOr(&L.ovf[q], 1u);
+}
+
+// The widening of a hazard box: 2^-14 of its coordinates' magnitude plus its size.
+__device__ __forceinline__ float widen(float4 b) {
+    const float mag = fmaxf(fmaxf(fabsf(b.x), fabsf(b.z)), fmaxf(fabsf(b.y), fabsf(b.w)));
+    return (mag + (b.z - b.x) + (b.w - b.y)) * 0x1p-14f;
+}
+__device__ __forceinline__ float4 widened(float4 b) {
+    const float w = widen(b);
+    return make_float4(b.x - w, b.y - w, b.z + w, b.w + w);
+}
+// The container's inner box shrunk by the margin (empty when there is none).
+__device__ __forceinline__ float4 inner_shrunk(const CdeView& c) {
+    const float4 b = c.ext_inner;
+    if (!(b.x < b.z)) return b;
+    const float w = widen(b);
+    return make_float4(b.x + w, b.y + w, b.z - w, b.w - w);
+}
+__device__ __forceinline__ bool boxes_meet(float4 a, float4 b) {
+    return a.x <= b.z && a.z >= b.x && a.y <= b.w && a.w >= b.y;
+}
+__device__ __forceinline__ bool box_inside(float4 in, float4 out) {
+    return in.x > out.x && in.z < out.z && in.y > out.y && in.w < out.w;
+}
+// Whether an entity with box eb may collect hazard `slot` anywhere below a node that holds it.
+__device__ __forceinline__ bool may_collect(const CdeView& c, uint32_t slot, float4 eb) {
+    if (slot == c.exterior_slot) return !box_inside(eb, inner_shrunk(c));
+    return boxes_meet(eb, widened(c.haz_bbox[slot]));
+}
+
+__device__ __forceinline__ float4 entity_box(bool circle, float4 g) {
+    return circle ? make_float4(g.x - g.z, g.y - g.z, g.x + g.z, g.y + g.z)
+                  : make_float4(mn(g.x, g.z), mn(g.y, g.w), mx(g.x, g.z), mx(g.y, g.w));
+}
+
+// The Partial entries of a detection-now node whose edges an entity tests (qt_node.rs:174-189:
+// the ff_bbox pre-test of qt_partial_hazard.rs:67-72).  The count and the pushes use this same
+// predicate, so they agree whatever the sets do meanwhile.
+__device__ __forceinline__ bool tests_edges(const DevEntry& e, bool circle, float4 g) {
+    return !(e.slot_flags & ENTIRE_BIT) && ff_pass(e, circle, g);
+}
+
+// This lane tests an entry's edges itself and inserts its slot on a hit (queue overflow).
+__device__ __forceinline__ void test_inline(const CdeView& c, CollectLds& L, const DevNode& nd, uint32_t qs,
+                                            bool circle, float4 g) {
+    const uint32_t he = nd.haz_begin + (nd.haz_info & COUNT_MASK);
+    for (uint32_t i = nd.haz_begin; i < he; i++) {
+        const DevEntry e = c.ent[i];
+        if (!tests_edges(e, circle, g)) continue;
+        const uint32_t sl = e.slot_flags & SLOT_MASK;
+        if (set_has(L, qs, sl)) continue;
+        for (uint32_t j = 0; j < e.edge_count; j++)
+            if (edge_hit(circle, g, c.edges[e.edge_begin + j])) {
+                set_put(L, qs, sl);
+                break;
+            }
+    }
+}
+
+// Whether a visit of a pole or pier to node nd may still add to its query's set: some entry
+// outside the set may be collected by the entity's box (entries read four at a time).
+__device__ __forceinline__ bool surrogate_live(const CdeView& c, const CollectLds& L, const DevNode& nd, uint32_t cnt,
+                                               uint32_t qs, float4 eb) {
+    for (uint32_t i0 = 0; i0 < cnt; i0 += 4) {
+        uint32_t sl[4];
+#pragma unroll
+        for (uint32_t r = 0; r < 4; r++) sl[r] = c.ent[nd.haz_begin + (i0 + r < cnt ? i0 + r : 0u)].slot_flags & SLOT_MASK;
+        float4 hb[4];
+#pragma unroll
+        for (uint32_t r = 0; r < 4; r++) hb[r] = c.haz_bbox[sl[r]];
+#pragma unroll
+        for (uint32_t r = 0; r < 4; r++) {
+            if (i0 + r >= cnt || set_has(L, qs, sl[r])) continue;
+            if (sl[r] == c.exterior_slot ? !box_inside(eb, inner_shrunk(c)) : boxes_meet(eb, widened(hb[r]))) return true;
+        }
+    }
+    return false;
+}
+
+// Runs the wave's nn pending node visits of QTNode::collect_collisions and the edge tests they
+// create to completion, inserting into the queries' sets.  Same queue discipline as the
+// detection's traversal (the node queue cannot overflow; an edge push that would overflow is
+// tested inline).
+__device__ __forceinline__ void wave_collect(const CdeView& c, CollectLds& L, uint32_t nn, uint32_t lane) {
+    uint32_t ne = 0;
+    while (nn + ne != 0u) {
+        wave_sync();
+        if (ne >= 64u || nn == 0u) {
+            const uint32_t take = ne < 64u ? ne : 64u;
+            if (lane < take) {
+                const uint32_t at = ne - 1u - lane;
+                const uint32_t o = L.edge_ent[at], sl = L.eslot[at];
+                const uint32_t es = L.ent_slot[o], qs = es & 63u;
+                const float4 he = c.edges[L.edges[at]];
+                if (!L.ovf[qs] && !set_has(L, qs, sl) && edge_hit((es & WL_CIRCLE) != 0u, L.geom[o], he))
+                    set_put(L, qs, sl);
+            }
+            ne -= take;
+            continue;
+        }
+        uint32_t take = 1u;
+        if (nn + 3u + WL_RESERVE <= WL_NCAP) {
+            const uint32_t room = (WL_NCAP - WL_RESERVE - nn) / 3u;
+            take = nn < 64u ? nn : 64u;
+            take = take < room ? take : room;
+        }
+        const uint32_t base = nn - take;
+        const uint32_t item = lane < take ? L.nodes[nn - 1u - lane] : 0u;
+        const uint32_t o = item >> 24;
+        uint32_t qs = 0, em = 0;
+        bool live = false;
+        if (lane < take) {
+            qs = L.ent_slot[o] & 63u;
+            em = L.emask[o];
+            // a query edge whose candidates are all collected adds nothing below any node
+            live = !L.ovf[qs] && (em == SURR_MASK || (em & ~L.cdone[qs]) != 0u);
+        }
+        const DevNode nd = c.nodes[live ? (item & WL_NODE_MASK) : 0u];
+        uint32_t m = 0, n_edges = 0;
+        int fc = 0;
+        bool circle = false;
+        float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const uint32_t cnt = nd.haz_info & COUNT_MASK;
+        if (live) {
+            const uint32_t es = L.ent_slot[o];
+            circle = (es & WL_CIRCLE) != 0u;
+            g = L.geom[o];
+            live = cnt > 0u;
+            if (live && em == SURR_MASK && cnt <= PRUNE_MAX) live = surrogate_live(c, L, nd, cnt, qs, entity_box(circle, g));
+            if (live) {
+                if (nd.first_child >= 0 && nd.n_active_edges > c.cd_threshold) {
+                    fc = nd.first_child;
+                    if (circle) {
+                        for (int k = 0; k < 4; k++) m |= circle_box(g.x, g.y, g.z, box_quadrant(nd.bbox, k)) ? (1u << k) : 0u;
+                    } else {
+                        m = edge_quadrants(g, nd.bbox);
+                    }
+                } else {
+                    for (uint32_t i = nd.haz_begin; i < nd.haz_begin + cnt; i++) {
+                        const DevEntry e = c.ent[i];
+                        const uint32_t sl = e.slot_flags & SLOT_MASK;
+                        if (e.slot_flags & ENTIRE_BIT) {
+                            if (!set_has(L, qs, sl)) set_put(L, qs, sl);
+                        } else if (ff_pass(e, circle, g)) {
+                            n_edges += e.edge_count;
+                        }
+                    }
+                    if (n_edges >= CE_LIMIT) {
+                        test_inline(c, L, nd, qs, circle, g);
+                        n_edges = 0;
+                    }
+                }
+            }
+        }
+        const uint32_t packed = (uint32_t)__builtin_popcount(m) | (n_edges << 16);
+        const uint32_t incl = wave_incl_scan(packed, lane);
+        const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+        const uint32_t excl = incl - packed;
+        wave_sync();                              // every lane has read its item before pushes overwrite
+        {
+            uint32_t at = base + (excl & 0xffffu);
+            for (uint32_t b = m; b; b &= b - 1u) L.nodes[at++] = (item & ~WL_NODE_MASK) | (uint32_t)(fc + __builtin_ctz(b));
+            nn = base + (tot & 0xffffu);
+        }
+        const uint32_t e_tot = tot >> 16;
+        const bool fits = ne + e_tot <= CE_CAP;
+        if (n_edges) {
+            if (fits) {
+                uint32_t at = ne + (excl >> 16);
+                for (uint32_t i = nd.haz_begin; i < nd.haz_begin + cnt; i++) {
+                    const DevEntry e = c.ent[i];
+                    if (!tests_edges(e, circle, g)) continue;
+                    const uint32_t sl = e.slot_flags & SLOT_MASK;
+                    for (uint32_t j = 0; j < e.edge_count; j++, at++) {
+                        L.edges[at] = e.edge_begin + j;
+                        L.eslot[at] = sl;
+                        L.edge_ent[at] = (uint8_t)o;
+                    }
+                }
+            } else {
+                test_inline(c, L, nd, qs, circle, g);
+            }
+        }
+        if (fits) ne += e_tot;
+    }
+    wave_sync();
+}
+
+__device__ __forceinline__ uint32_t push_root(CollectLds& L, uint32_t n, bool want, uint32_t ent, uint32_t node,
+                                              uint32_t lane) {
+    const uint64_t b = __ballot(want);
+    if (want) L.nodes[n + lanes_below(b, lane)] = (ent << 24) | node;
+    return n + (uint32_t)__builtin_popcountll(b);
+}
+
+__device__ __forceinline__ uint64_t group_sum_u64(uint64_t v) {
+    for (uint32_t o = CG >> 1; o > 0; o >>= 1) {
+        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, (int)o, 64);
+        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), (int)o, 64);
+        v += ((uint64_t)hi << 32) | lo;
+    }
+    return v;
+}
+
+// detect_containment_collision (cd_engine.rs:231-255) for one candidate of the query, by the
+// query's CG lanes (the one holding it in `pend`): inserts it when it collides.
+__device__ __forceinline__ void containment_group(const CdeView& c, CollectLds& L, const ItemView& it, const Mat& m,
+                                                  float4 bb, float2 qpoi, uint32_t qs, uint32_t sub, uint32_t lane,
+                                                  bool pend, uint32_t sl, int rel) {
+    const uint64_t gmask = (1ull << CG) - 1ull;
+    const uint32_t gbase = qs * CG;
+    for (uint64_t pm = (ballot(pend) >> gbase) & gmask; pm; pm &= pm - 1ull) {
+        const int l = (int)gbase + __builtin_ctzll(pm);
+        const uint32_t s2 = (uint32_t)__shfl((int)sl, l, 64);
+        const int rl = __shfl(rel, l, 64);
+        bool contained;
+        if (rl == REL_SURROUNDING) {
+            contained = pip_hazard_group(c, s2, qpoi.x, qpoi.y, sub, CG);
+        } else {
+            const float4 hp = c.haz_poi[s2];
+            contained = pip_query_group(it, m, bb, hp.x, hp.y, sub, CG);
+        }
+        if (lane == (uint32_t)l && (c.haz_poi[s2].w != 0.0f ? contained : !contained)) set_put(L, qs, s2);
+    }
+}
+
+// The hazard sets of queries q0 .. q0 + P - 1 of set sd (P <= CP).
+__device__ __forceinline__ void collect_pass(const SetDesc& sd, uint32_t set_index, uint32_t q0, uint32_t P,
+                                             CollectLds& L, const float* __restrict__ m6, uint32_t cap,
+                                             uint32_t* __restrict__ count, uint64_t* __restrict__ hash,
+                                             uint32_t* __restrict__ keys, uint32_t* __restrict__ counters,
+                                             uint64_t* __restrict__ ovf, uint32_t lane) {
+    const CdeView c = sd.cde;
+    const ItemView it = sd.item;
+    const uint32_t qs = lane / CG, sub = lane % CG;
+    const bool active = qs < P;
+    const uint64_t out = sd.first + q0 + qs;
+    Mat m{};
+    if (active) m = load_mat(m6, out);
+    // the transformed bbox (simple_polygon.rs:122-133), reduced over the query's CG lanes
+    float4 bb = make_float4(3.40282347e+38f, 3.40282347e+38f, -3.40282347e+38f, -3.40282347e+38f);
+    if (active)
+        for (uint32_t i = sub; i < it.n_v; i += CG) {
+            const float2 p = xform(m, it.v[i].x, it.v[i].y);
+            bb.x = mn(bb.x, p.x); bb.y = mn(bb.y, p.y); bb.z = mx(bb.z, p.x); bb.w = mx(bb.w, p.y);
+        }
+    for (uint32_t o = CG >> 1; o > 0; o >>= 1) {
+        bb.x = fminf(bb.x, __shfl_xor(bb.x, (int)o, 64));
+        bb.y = fminf(bb.y, __shfl_xor(bb.y, (int)o, 64));
+        bb.z = fmaxf(bb.z, __shfl_xor(bb.z, (int)o, 64));
+        bb.w = fmaxf(bb.w, __shfl_xor(bb.w, (int)o, 64));
+    }
+    for (uint32_t i = sub; i < SET_CAP; i += CG) L.set[qs * SET_CAP + i] = 0u;
+    if (sub == 0) {
+        L.ovf[qs] = 0u;
+        L.ncand[qs] = 0u;
+        L.cdone[qs] = 0u;
+    }
+    int vr = 0;
+    if (active) vr = virtual_root(c, bb);
+    wave_sync();
+    // the candidates: the virtual root's entries whose widened box meets the query's bbox, and
+    // the exterior unless the bbox is inside the inner box; four entries in flight per lane
+    uint32_t vcnt = 0, ve0 = 0;
+    if (active) {
+        const DevNode vn = c.nodes[vr];
+        ve0 = vn.haz_begin;
+        vcnt = vn.haz_info & COUNT_MASK;
+        const float4 inner = inner_shrunk(c);
+        for (uint32_t k0 = 0; k0 < vcnt; k0 += 4u * CG) {
+            uint32_t sf[4];
+#pragma unroll
+            for (uint32_t r = 0; r < 4; r++) {
+                const uint32_t k = k0 + r * CG + sub;
+                sf[r] = c.ent[ve0 + (k < vcnt ? k : 0u)].slot_flags;
+            }
+            float4 hb[4];
+#pragma unroll
+            for (uint32_t r = 0; r < 4; r++) hb[r] = c.haz_bbox[sf[r] & SLOT_MASK];
+#pragma unroll
+            for (uint32_t r = 0; r < 4; r++) {
+                if (k0 + r * CG + sub >= vcnt) continue;
+                const uint32_t sl = sf[r] & SLOT_MASK;
+                const bool ext = sl == c.exterior_slot;
+                const float4 wb = ext ? inner : widened(hb[r]);
+                if (ext ? box_inside(bb, inner) : !boxes_meet(bb, wb)) continue;
+                const uint32_t at = atomicAdd(&L.ncand[qs], 1u);
+                if (at < NC) {
+                    L.cand[qs * NC + at] = sl | (ext ? C_EXT : 0u) | ((sf[r] & ENTIRE_BIT) ? C_ENTIRE : 0u);
+                    L.cbox[qs * NC + at] = wb;
+                }
+            }
+        }
+    }
+    wave_sync();
+    if (active) {
+        if (sub == 0) L.vroot[qs] = vr;
+        // a pre-seeded collector, then the exterior when the shape leaves the engine's bbox
+        // (cd_engine.rs:262-264); the traversal goes on regardless
+        for (uint32_t i = sub; i < sd.n_seed; i += CG) set_put(L, qs, sd.seed[i]);
+        if (sub == 0 && box_relation(c.nodes[0].bbox, bb) != REL_SURROUNDING) set_put(L, qs, c.exterior_slot);
+    }
+    // every query's ff poles / piers from the root and edges from its virtual root, WL_ENT
+    // entities per traversal
+    const uint32_t n_ff = it.n_poles + it.n_piers;
+    const uint32_t n_ent = n_ff + it.n_v;
+    const uint32_t total = P * n_ent;
+    for (uint32_t e0 = 0; e0 < total; e0 += WL_ENT) {
+        wave_sync();
+        uint32_t n = 0;
+        for (uint32_t j = 0; j < WL_ENT; j += WAVE) {
+            const uint32_t o = j + lane;
+            const uint32_t e = e0 + o;
+            bool want = false;
+            uint32_t root = 0;
+            if (e < total) {
+                const uint32_t qi = e / n_ent, k = e - qi * n_ent;
+                if (!L.ovf[qi]) {
+                    const Mat mq = load_mat(m6, sd.first + q0 + qi);
+                    bool circle = false;
+                    float4 gm;
+                    uint32_t em = SURR_MASK;
+                    if (k < n_ff) {
+                        const float4 p = it.ff[k];
+                        circle = k < it.n_poles;
+                        if (circle) {
+                            const float2 cc = xform(mq, p.x, p.y);
+                            gm = make_float4(cc.x, cc.y, p.z, 0.0f);
+                        } else {
+                            const float2 a = xform(mq, p.x, p.y), b = xform(mq, p.z, p.w);
+                            gm = make_float4(a.x, a.y, b.x, b.y);
+                        }
+                    } else {
+                        const uint32_t i = k - n_ff, i1 = i + 1 == it.n_v ? 0 : i + 1;
+                        const float2 a = xform(mq, it.v[i].x, it.v[i].y), b = xform(mq, it.v[i1].x, it.v[i1].y);
+                        gm = make_float4(a.x, a.y, b.x, b.y);
+                        root = (uint32_t)L.vroot[qi];
+                        // the candidates this edge's box may collect (all of them when the
+                        // query has more than NC)
+                        const uint32_t nc = L.ncand[qi];
+                        if (nc > NC) {
+                            em = 0xffffffffu;
+                        } else {
+                            const float4 eb = entity_box(false, gm);
+                            em = 0u;
+                            for (uint32_t t = 0; t < nc; t++) {
+                                const uint32_t cf = L.cand[qi * NC + t];
+                                const float4 cb = L.cbox[qi * NC + t];
+                                const bool may = (cf & C_EXT) ? !box_inside(eb, cb) : boxes_meet(eb, cb);
+                                em |= may ? (1u << t) : 0u;
+                            }
+                        }
+                        // an edge that may collect nothing is not pushed
+                        want = em != 0u;
+                    }
+                    if (k < n_ff) want = true;
+                    if (want) {
+                        L.geom[o] = gm;
+                        L.emask[o] = em;
+                        L.ent_slot[o] = (uint8_t)(qi | (circle ? WL_CIRCLE : 0u));
+                    }
+                }
+            }
+            n = push_root(L, n, want, o, root, lane);
+        }
+        wave_collect(c, L, n, lane);
+    }
+    // containment over the virtual root's Partial hazards not yet collected
+    // (cd_engine.rs:276-287), by the query's CG lanes: a candidate without bbox relation
+    // collides iff its scope is exterior; the point-in-polygon tests left are split over the
+    // CG lanes (containment()'s result).  Only candidates can collide; with more than NC the
+    // virtual root's entries are scanned.
+    if (active && !L.ovf[qs]) {
+        const float2 qpoi = xform(m, it.poi.x, it.poi.y);
+        const uint32_t nc = L.ncand[qs];
+        const bool listed = nc <= NC;
+        const uint32_t n_scan = listed ? nc : vcnt;
+        for (uint32_t k0 = 0; k0 < n_scan; k0 += CG) {
+            const uint32_t k = k0 + sub;
+            uint32_t sl = 0;
+            int rel = REL_DISJOINT;
+            bool pend = false;
+            if (k < n_scan) {
+                uint32_t sf;
+                if (listed) {
+                    const uint32_t cf = L.cand[qs * NC + k];
+                    sf = (cf & C_SLOT) | ((cf & C_ENTIRE) ? ENTIRE_BIT : 0u);
+                } else {
+                    sf = c.ent[ve0 + k].slot_flags;
+                }
+                sl = sf & SLOT_MASK;
+                if (!(sf & ENTIRE_BIT) && !set_has(L, qs, sl)) {
+                    rel = box_almost_relation(c.haz_bbox[sl], bb);
+                    if (rel == REL_SURROUNDING || rel == REL_ENCLOSED) pend = true;
+                    else if (c.haz_poi[sl].w == 0.0f) set_put(L, qs, sl);
+                }
+            }
+            containment_group(c, L, it, m, bb, qpoi, qs, sub, lane, pend, sl, rel);
+        }
+    }
+    wave_sync();
+    // results: count, order-independent hash, the first `cap` slots ascending
+    if (active) {
+        const uint32_t* t = L.set + qs * SET_CAP;
+        if (L.ovf[qs]) {
+            if (sub == 0) {
+                const uint32_t at = atomicAdd(&counters[C_OVERFLOW], 1u);
+                ovf[at] = ((uint64_t)set_index << 32) | (uint32_t)(q0 + qs);
+            }
+        } else {
+            uint32_t n = 0;
+            while (n < SET_CAP && t[n] != 0u) n++;
+            uint64_t h = 0;
+            for (uint32_t i = sub; i < n; i += CG) {
+                const uint32_t si = t[i] - 1u;
+                h += mix64((uint64_t)si);
+                if (keys && cap) {
+                    uint32_t rank = 0;
+                    for (uint32_t j = 0; j < n; j++) rank += t[j] - 1u < si ? 1u : 0u;
+                    if (rank < cap) keys[out * cap + rank] = si;
+                }
+            }
+            if (keys)
+                for (uint32_t k = n + sub; k < cap; k += CG) keys[out * cap + k] = 0xffffffffu;
+            h = group_sum_u64(h);
+            if (sub == 0) {
+                count[out] = n;
+                hash[out] = h;
+            }
+        }
+    }
+    wave_sync();
+}
+
+__global__ void k_collect_reset(uint32_t* counters) {
+    for (uint32_t i = threadIdx.x; i < N_COUNTER_WORDS; i += blockDim.x) counters[i] = 0u;
+}
+
+__global__ __launch_bounds__(WAVE) void k_collect_units(const SetDesc* __restrict__ sets, const Unit* __restrict__ units,
+                                                       uint32_t n_units, uint32_t* __restrict__ counters,
+                                                       const float* __restrict__ m6, uint32_t cap,
+                                                       uint32_t* __restrict__ count, uint64_t* __restrict__ hash,
+                                                       uint32_t* __restrict__ keys, uint64_t* __restrict__ ovf) {
+    __shared__ CollectLds lds;
+    const uint32_t lane = threadIdx.x;
+    uint32_t ux = blockIdx.x & (N_STREAMS - 1u), utried = 0;
+    for (;;) {
+        const uint32_t u = claim_next(counters, C_USTREAM0, ux, utried, n_units, lane);
+        if (u >= n_units) return;
+        const Unit un = units[u];
+        const uint32_t si = wave_bcast(un.set);
+        for (uint32_t p0 = 0; p0 < un.count; p0 += CP) {
+            const uint32_t P = un.count - p0 < CP ? un.count - p0 : CP;
+            collect_pass(sets[si], si, un.q0 + p0, P, lds, m6, cap, count, hash, keys, counters, ovf, lane);
+        }
+    }
+}
+
+}  // namespace
+
+namespace jg {
+
+jg_status collect_launch(const TableLayout& L, uint8_t* base, const float* d_m6, uint32_t cap, uint32_t* d_count,
+                         uint64_t* d_hash, uint32_t* d_keys, uint64_t* ovf, uint32_t n_cus, hipStream_t stream) {
+    uint32_t* counters = (uint32_t*)(base + L.o_ctr);
+    const SetDesc* sets = (const SetDesc*)base;
+    const Unit* units = (const Unit*)(base + L.o_units);
+    hipLaunchKernelGGL(k_collect_reset, dim3(1), dim3(64), 0, stream, counters);
+    const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(n_cus * 16u, L.n_units));
+    hipLaunchKernelGGL(k_collect_units, dim3(grid), dim3(WAVE), 0, stream, sets, units, L.n_units, counters, d_m6,
+                       d_keys ? cap : 0u, d_count, d_hash, d_keys, ovf);
+    jg_status st = hip_check(hipGetLastError(), "batched collection launch");
+    if (st != JG_OK) return st;
+    collect_overflow_launch(sets, counters, ovf, d_m6, L.max_slot_words, d_keys ? cap : 0u, d_count, d_hash, d_keys,
+                            n_cus * 4u, stream);
+    return hip_check(hipGetLastError(), "collection overflow launch");
+}
+
+}  // namespace jg

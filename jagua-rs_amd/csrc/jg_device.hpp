@@ -34,6 +34,9 @@ struct CdeView {
     const uint2* haz_vr;
     const float2* haz_verts;
     uint32_t n_slots, exterior_slot, cd_threshold, n_nodes;
+    float4 ext_inner;                 // a box inside the container (the exterior hazard's shape):
+                                      // its bbox when that shape is an axis-aligned rectangle,
+                                      // else empty (x_min > x_max)
 };
 
 struct ItemView {

@@ -30,6 +30,7 @@ struct jg_spolygon {
     jg::DevBlob dev;                   // vertices, ff poles, piers
     jg::ItemView view{};
     bool dev_valid = false;
+    uint64_t upload_gen = 0;           // bumped on every upload (a batched plan's table refers to the upload)
 };
 
 struct jg_cde {
@@ -45,9 +46,13 @@ struct jg_cde {
     std::vector<uint8_t> staging;
     std::vector<jg::ImageSeg> segs;
     jg::DevBlob filter_dev;            // JG_FILTER_KEYS mask
+    hipEvent_t filter_read = nullptr;  // completion of the last launch that read filter_dev
     jg::CdeView view{};
     jg_cde_stats stats{};
     uint64_t generation = 0;           // bumped on every upload of a new image
+    ~jg_cde() {
+        if (filter_read) (void)hipEventDestroy(filter_read);
+    }
 };
 
 struct jg_snapshot {
@@ -57,7 +62,11 @@ struct jg_snapshot {
 namespace jg {
 jg_status cde_upload(jg_cde* c);
 jg_status item_upload(jg_spolygon* it);
-jg_status filter_view(jg_cde* c, const jg_hazard_filter* f, FilterView& out);
+// the device record of a query item: its size, and its bytes written for device address dev
+size_t item_image_bytes(const Polygon& p);
+void item_image_write(const Polygon& p, uint8_t* host, uint8_t* dev, ItemView& view);
+jg_status filter_view(jg_cde* c, const jg_hazard_filter* f, FilterView& out, hipStream_t stream);
+jg_status filter_read_done(jg_cde* c, hipStream_t stream);
 jg_status plan_prepare(jg_plan* p);
 jg_status plan_enqueue(jg_plan* p, const float* d_m6, uint8_t* d_status, hipStream_t stream);
 // Piers of many polygons on the GPU (generate_piers, fail_fast/piers.rs:25-98), bit for bit

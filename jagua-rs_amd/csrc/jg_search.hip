@@ -180,7 +180,7 @@ jg_status search_prepare(jg_search* S) {
     std::vector<uint64_t> gens;                       // engine images and item uploads the buffers refer to
     for (auto& it : S->items) {
         gens.push_back(it.cde->generation);
-        gens.push_back((uint64_t)(uintptr_t)it.item->view.v);
+        gens.push_back(it.item->upload_gen);
     }
     if (gens == S->generations && S->blob.ptr) return JG_OK;
     const uint32_t n = (uint32_t)S->items.size(), K = S->cfg.samples_per_round;

@@ -309,8 +309,9 @@ def _is_torch(x):
 
 
 def _stream_ptr(stream):
+    """A hipStream_t: a torch stream's handle, a raw handle, or torch's current stream."""
     if stream is not None:
-        return C.c_void_p(stream)
+        return C.c_void_p(getattr(stream, "cuda_stream", stream))
     import torch
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
@@ -441,6 +442,21 @@ class CDEngine:
         return cnt, hsh, keys
 
 
+def _query_sets(sets):
+    """jg_query_set array of [(engine, item, first, count, filter)] and the objects it refers to."""
+    arr = (N.QuerySetC * len(sets))()
+    keep = []
+    for i, (cde, item, first, count, filt) in enumerate(sets):
+        arr[i].cde = cde._h.value
+        arr[i].item = item._h.value
+        arr[i].first = first
+        arr[i].count = count
+        f = (filt or NoFilter())
+        keep.append(f)
+        arr[i].filter = f._filter()
+    return arr, keep
+
+
 class QueryPlan:
     """Batched persistent form: many (engine, item, transform range) sets in one launch.
 
@@ -452,16 +468,7 @@ class QueryPlan:
 
     def __init__(self, sets, chunk=0, workgroups=0, lanes_per_query=0, flags=0, stages=0):
         n = len(sets)
-        arr = (N.QuerySetC * n)()
-        self._keep = []
-        for i, (cde, item, first, count, filt) in enumerate(sets):
-            arr[i].cde = cde._h.value
-            arr[i].item = item._h.value
-            arr[i].first = first
-            arr[i].count = count
-            f = (filt or NoFilter())
-            self._keep.append(f)
-            arr[i].filter = f._filter()
+        arr, self._keep = _query_sets(sets)
         cfg = N.PlanConfigC(chunk, workgroups, lanes_per_query, flags, stages)
         self._h = C.c_void_p()
         self._refs = [s[0] for s in sets] + [s[1] for s in sets]
@@ -481,6 +488,22 @@ class QueryPlan:
         check(N.lib().jg_plan_detect_graph(self._h, C.c_void_p(mats.data_ptr()), C.c_void_p(out.data_ptr()),
                                            replays, _stream_ptr(stream)))
         return out
+
+    def collect(self, mats, capacity=16, count=None, hash=None, keys=None, stream=None):
+        """Hazard sets of every query (jg_plan_collect): (count u32 [n], hash as int64 [n],
+        keys int32 [n, capacity] ascending, padded with -1) as torch tensors on the device.
+        A set's HazKeyFilter is a pre-seeded collector: its keys are in every set."""
+        import torch
+        n = mats.shape[0]
+        dev = mats.device
+        count = count if count is not None else torch.empty(n, dtype=torch.int32, device=dev)
+        hash = hash if hash is not None else torch.empty(n, dtype=torch.int64, device=dev)
+        if capacity and keys is None:
+            keys = torch.empty((n, capacity), dtype=torch.int32, device=dev)
+        check(N.lib().jg_plan_collect(self._h, C.c_void_p(mats.data_ptr()), capacity, C.c_void_p(count.data_ptr()),
+                                      C.c_void_p(hash.data_ptr()),
+                                      C.c_void_p(keys.data_ptr()) if capacity else None, _stream_ptr(stream)))
+        return count, hash, keys
 
     def autotune(self, mats, out, reps=3, stream=None):
         """Times candidate schedules on these inputs and keeps the fastest (jg_plan_autotune).

@@ -254,8 +254,10 @@ def make_filter(kind=0, seed=0, threshold=0, keys=()):
 
 
 def query(cde, item, mats=None, dts=None, mode=3, filt=None, keys_cap=0, threads=1, counters=False):
-    """Runs n collision queries on the oracle.  Returns a dict with status, surr, count,
-    hash, keys (when requested), counters (nodes, entries, edges, pip vertices), seconds."""
+    """Runs n collision queries on the oracle.  mode: 1 detect, 2 collect, 3 both; | 4 seeds
+    the collector with a key filter's slots (a pre-seeded collector).  Returns a dict with
+    status, surr, count, hash, keys (when requested), counters (nodes, entries, edges, pip
+    vertices), seconds."""
     src = _f(mats).reshape(-1, 6) if mats is not None else _f(dts).reshape(-1, 3)
     n = len(src)
     status = np.zeros(n, dtype=np.uint8)

@@ -428,6 +428,11 @@ int or_query_batch(void* cde_p, void* item_p, const float* dt, const float* mat,
                 }
                 if (mode & 2) {
                     col.reset(n_slots);
+                    // mode bit 4: the collector arrives pre-seeded with the key filter's slots
+                    // (a collector used as the filter, filter.rs:69-76)
+                    if ((mode & 4) && filt && filt->kind == 1)
+                        for (uint32_t i = 0; i < filt->n_keys; i++)
+                            if (filt->keys[i] < n_slots) col.insert(HazKey{filt->keys[i], 0});
                     cde.collect_surrogate_collisions(item.shape.surrogate, t, col, ct);
                     do_transform();
                     cde.collect_poly_collisions(buffer, col, ct);

@@ -235,3 +235,23 @@ def test_depth0_equals_deeper_trees(which):
                 print(name, "depth mismatch at sample", q, "robust" if robust else "within TOL of a boundary")
     assert robust_mismatch == 0
     assert mismatched <= 1e-3 * total, (mismatched, total)
+
+
+def test_preseeded_collector_is_the_union():
+    """A collector that arrives holding some keys (the collector as filter, filter.rs:69-76)
+    ends with those keys together with the unseeded set: collection only adds
+    (collector.rs:44-67), so seeding can change nothing else."""
+    name, sc = scenes()[1]
+    oc = O.OracleCDE(sc)
+    hz = hazards_of(sc, oc)
+    n_slots = max(h.slot for h in hz) + 1
+    seed = [h.slot for h in hz][1::4]
+    for p, mats in queries(21):
+        oit = O.OracleItem(p)
+        plain = O.query(oc, oit, mats=mats, mode=2, keys_cap=n_slots, threads=8)
+        seeded = O.query(oc, oit, mats=mats, mode=2 | 4, filt=O.make_filter(1, keys=seed), keys_cap=n_slots,
+                         threads=8)
+        for q in range(len(mats)):
+            a = set(plain["keys"][q][: plain["count"][q]].tolist())
+            b = set(seeded["keys"][q][: seeded["count"][q]].tolist())
+            assert b == a | set(seed), (q, sorted(a), sorted(b))
