@@ -293,6 +293,26 @@ jg_status jg_plan_autotune(jg_plan* plan, const float* d_mat6, uint8_t* d_status
  * means per item). */
 jg_status jg_plan_get_config(const jg_plan* plan, jg_plan_config* out);
 
+/* The batched detection of jg_plan_create + jg_plan_detect in one call that allocates no
+ * memory.  No reference counterpart: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The sets, schedule, verdicts and kernels are jg_plan_detect's.  Instead of a plan's own
+ * buffers it uses one fixed device array of JG_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends, for the launch table (set descriptors, work units, claim counters, survivor
+ * groups), the query items' records and JG_FILTER_KEYS masks; the caller passes the
+ * transforms and the status array.  Requirements and limits:
+ *  - every engine's image must be current on the device (jg_cde_sync after the last update;
+ *    JG_ERR_INVALID_ARG otherwise), since uploading an image allocates;
+ *  - a call whose table outgrows the scratch returns JG_ERR_CAPACITY (about 8 bytes per query
+ *    plus 16 per survivor pass: roughly 2M queries of configuration-2 items fit);
+ *  - every call on one GPU shares that GPU's scratch: calls that overlap on one GPU (other
+ *    streams or threads, before the previous call's kernels finished) race.  Calls in order
+ *    on one stream are safe: the staging copy is ordered after the previous call's kernels.
+ * The staging copy is from pageable host memory, so the call returns once it is staged. */
+#define JG_NOALLOC_SCRATCH_BYTES (32u << 20)
+jg_status jg_detect_sets_noalloc(const jg_query_set* sets, uint32_t n_sets, const jg_plan_config* cfg,
+                                 const float* d_m6, uint8_t* d_status, void* stream);
+
 /* ---------------------------------------------------------------- placement search */
 
 /* lbf's sampled placement search (lbf/src/opt/search.rs:17-106), resident on the GPU for many

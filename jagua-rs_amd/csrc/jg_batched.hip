@@ -20,6 +20,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -850,6 +851,120 @@ extern "C" jg_status jg_plan_detect_graph(jg_plan* p, const float* d_m6, uint8_t
         if (st != JG_OK) return st;
     }
     return JG_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// The batched detection without any allocation (jg_detect_sets_noalloc): the launch table,
+// the query items' records and the JG_FILTER_KEYS masks are staged into `scratch`, a fixed
+// device array of JG_NOALLOC_SCRATCH_BYTES that every call on a device shares.  Host code
+// never passes the array's name to the device: `scratch` names a host-side shadow, and its
+// device address comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) uint8_t scratch[JG_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_ptr[MAX_DEVICES] = {};
+uint32_t g_scratch_cus[MAX_DEVICES] = {};
+
+// The device address of this device's `scratch` (looked up on the first call on the device,
+// with that device current) and its compute-unit count.
+jg_status scratch_of_device(uint8_t** out, uint32_t* cus) {
+    int dev = 0;
+    jg_status st = hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (st != JG_OK) return st;
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(JG_ERR_INTERNAL, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        st = hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (st != JG_OK) return st;
+        if (!p) return fail(JG_ERR_HIP, "hipGetSymbolAddress(scratch) returned no address");
+        g_scratch_ptr[dev] = (uint8_t*)p;
+        g_scratch_cus[dev] = device_cus();
+    }
+    *out = g_scratch_ptr[dev];
+    *cus = g_scratch_cus[dev];
+    return JG_OK;
+}
+}  // namespace
+
+extern "C" jg_status jg_detect_sets_noalloc(const jg_query_set* sets, uint32_t n_sets, const jg_plan_config* cfg,
+                                            const float* d_m6, uint8_t* d_status, void* stream) {
+    if (n_sets && !sets) return fail(JG_ERR_INVALID_ARG, "null argument");
+    jg_plan_config pc{};
+    if (cfg) pc = *cfg;
+    const uint32_t flags = pc.flags ? pc.flags : (JG_QUERY_SURROGATE | JG_QUERY_POLY);
+    std::vector<jg_query_set> vs;
+    std::vector<jg_hazard_filter> filters;
+    std::vector<std::vector<jg_hazkey>> keys;
+    uint64_t total_q = 0;
+    for (uint32_t i = 0; i < n_sets; i++) {
+        const jg_query_set& s = sets[i];
+        if (!s.cde || !s.item) return fail(JG_ERR_INVALID_ARG, "null engine or item in a query set");
+        if (s.count > 0xffffffffull) return fail(JG_ERR_INVALID_ARG, "query set too large");
+        const Polygon& poly = s.item->poly;
+        if (flags & JG_QUERY_SURROGATE) {
+            if (!poly.sur.present) return fail(JG_ERR_SURROGATE, "surrogate not generated");
+            if (poly.sur.cfg.n_ff_poles > poly.sur.poles.size()) return fail(JG_ERR_SURROGATE, "ff_poles slice out of range");
+        }
+        if (s.filter.kind > JG_FILTER_HASHED) return fail(JG_ERR_INVALID_ARG, "unknown filter kind");
+        // an engine image is uploaded (allocated) by jg_cde_sync, never here
+        if (s.cde->dirty || !s.cde->dev_valid)
+            return fail(JG_ERR_INVALID_ARG, "engine image not current on the device: call jg_cde_sync first");
+        vs.push_back(s);
+        filters.push_back(s.filter);
+        keys.emplace_back();
+        if (s.filter.kind == JG_FILTER_KEYS && s.filter.n_keys) {
+            if (!s.filter.keys) return fail(JG_ERR_INVALID_ARG, "null filter keys");
+            keys.back().assign(s.filter.keys, s.filter.keys + s.filter.n_keys);
+        }
+        total_q += s.count;
+    }
+    if (total_q == 0) return JG_OK;
+    if (!d_m6 || !d_status) return fail(JG_ERR_INVALID_ARG, "null device buffer");
+    uint8_t* base = nullptr;
+    uint32_t cus = 0;
+    jg_status st = scratch_of_device(&base, &cus);
+    if (st != JG_OK) return st;
+    // scratch layout: item records | filter masks | launch table
+    std::map<const jg_spolygon*, size_t> item_off;
+    size_t off = 0;
+    for (auto& s : vs)
+        if (item_off.emplace(s.item, off).second) off += item_image_bytes(s.item->poly);
+    const size_t o_masks = off;
+    std::vector<uint32_t> mask_words;
+    std::vector<size_t> mask_off;
+    filter_masks(vs, filters, keys, mask_words, mask_off);
+    const size_t o_table = (o_masks + 4 * mask_words.size() + 255) & ~(size_t)255;
+    std::vector<uint8_t> host(o_table, 0);
+    std::vector<ItemView> iv(vs.size());
+    for (auto& kv : item_off) {
+        ItemView v{};
+        item_image_write(kv.first->poly, host.data() + kv.second, base + kv.second, v);
+        for (size_t i = 0; i < vs.size(); i++)
+            if (vs[i].item == kv.first) iv[i] = v;
+    }
+    if (!mask_words.empty()) std::memcpy(host.data() + o_masks, mask_words.data(), 4 * mask_words.size());
+    std::vector<FilterView> fv;
+    for (size_t i = 0; i < vs.size(); i++)
+        fv.push_back(set_filter(vs[i], filters[i], (const uint32_t*)(base + o_masks) + mask_off[i]));
+    std::vector<uint8_t> table;
+    TableLayout L;
+    st = table_image(vs, iv, fv, {}, pc, cus, table, L);
+    if (st != JG_OK) return st;
+    const uint64_t need = (uint64_t)o_table + L.total;
+    if (need > JG_NOALLOC_SCRATCH_BYTES)
+        return fail(JG_ERR_CAPACITY, "jg_detect_sets_noalloc needs " + std::to_string(need) +
+                                         " bytes of scratch; the fixed scratch holds " +
+                                         std::to_string((uint64_t)JG_NOALLOC_SCRATCH_BYTES));
+    host.resize(o_table + table.size());
+    std::memcpy(host.data() + o_table, table.data(), table.size());
+    // pageable source: the copy is complete when the call returns, and ordered on `stream`
+    hipStream_t hs = (hipStream_t)stream;
+    st = hip_check(hipMemcpyAsync(base, host.data(), host.size(), hipMemcpyHostToDevice, hs), "hipMemcpyAsync scratch");
+    if (st != JG_OK) return st;
+    return table_launch(L, base + o_table, d_m6, d_status, hs);
 }
 
 extern "C" jg_status jg_plan_collect(jg_plan* p, const float* d_m6, uint32_t capacity, uint32_t* d_count,

@@ -457,6 +457,22 @@ def _query_sets(sets):
     return arr, keep
 
 
+# JG_NOALLOC_SCRATCH_BYTES: the fixed per-GPU scratch of detect_sets_noalloc
+NOALLOC_SCRATCH_BYTES = 32 << 20
+
+
+def detect_sets_noalloc(sets, mats, out, chunk=0, workgroups=0, lanes_per_query=0, flags=0, stages=0,
+                        stream=None):
+    """The batched detection of QueryPlan(sets).detect(mats, out) in one call that allocates
+    no device memory (jg_detect_sets_noalloc): its tables live in a fixed per-GPU scratch
+    array.  Every engine must be synced (CDEngine.sync()) after its last update."""
+    arr, keep = _query_sets(sets)
+    cfg = N.PlanConfigC(chunk, workgroups, lanes_per_query, flags, stages)
+    check(N.lib().jg_detect_sets_noalloc(arr, len(sets), C.byref(cfg), C.c_void_p(mats.data_ptr()),
+                                         C.c_void_p(out.data_ptr()), _stream_ptr(stream)))
+    return out
+
+
 class QueryPlan:
     """Batched persistent form: many (engine, item, transform range) sets in one launch.
 

@@ -117,6 +117,8 @@ _SIGS = {
                                      C.POINTER(C.c_float)]),
     "jg_plan_get_config": (C.c_int32, [c_vp, C.POINTER(PlanConfigC)]),
     "jg_plan_collect": (C.c_int32, [c_vp, c_vp, C.c_uint32, c_vp, c_vp, c_vp, c_vp]),
+    "jg_detect_sets_noalloc": (C.c_int32, [C.POINTER(QuerySetC), C.c_uint32, C.POINTER(PlanConfigC), c_vp, c_vp,
+                                           c_vp]),
     "jg_search_create": (C.c_int32, [C.POINTER(SearchItemC), C.c_uint32, C.POINTER(SearchConfigC), C.POINTER(c_vp)]),
     "jg_search_free": (None, [c_vp]),
     "jg_search_run": (C.c_int32, [c_vp, C.c_uint32, c_vp]),
