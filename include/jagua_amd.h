@@ -385,7 +385,10 @@ jg_status jg_search_samples(jg_search* s, float* dt, uint8_t* status);
  * primitives are Circle::separation_distance for circles and for edges, circle.rs:184-229):
  *   Q = sum over candidate item pairs (a,b), poles i of a, j of b, with |c_i - c_j| < r_i + r_j,
  *         of (sqrt((r_i + r_j)^2) - |c_i - c_j|)^2
- *     + sum over poles i and container edges e with dist(c_i, e) < r_i of (r_i - dist(c_i, e))^2
+ *     + sum over poles i with D(c_i) < r_i of (r_i - D(c_i))^2, where D is the signed distance to
+ *         the container boundary (the container edges): the distance to the nearest edge,
+ *         positive inside (ray-crossing parity over the edges) and negative outside, so that a
+ *         pole whose centre has left the container is pushed back by r_i + |D|
  * where c_i = R(theta_a) p_i + (x_a, y_a) for item a's pose (x, y, theta) and local pole
  * (p_i, r_i).  Forward and backward are FP32 kernels; the backward recomputes the pole
  * transforms instead of storing them.  Partial sums are reduced in a fixed order
@@ -402,11 +405,39 @@ jg_status jg_overlap_fwd(jg_overlap* ov, const float* d_pose, float* d_q, void* 
 /* Backward: dQ/d(x, y, theta) per item into d_grad device [n_items][3]. */
 jg_status jg_overlap_bwd(jg_overlap* ov, const float* d_pose, float* d_grad, void* stream);
 void jg_overlap_free(jg_overlap* ov);
+/* Candidate pairs from the engine's quadtree, found on the GPU, replacing the overlap's pairs:
+ * item k is the hazard of slot item_slots[k] in cde, with bounding circle centre (x, y) of its
+ * pose d_pose[k] (device) and radius rad[k] (host).  For each item i the quadtree nodes meeting
+ * its circle's box are visited down to their leaves; a hazard present there that is an item
+ * k > i whose circle overlaps i's gives the pair (i, k).  Every pair of items whose shapes
+ * overlap is found (both are present in the leaves covering the overlap).  The pairs are in a
+ * fixed order (by i, then by slot), and so is the adjacency the backward pass gathers with.
+ * Synchronises `stream` (the pair count is read back); *n_pairs_out may be NULL. */
+jg_status jg_overlap_pairs_from_cde(jg_overlap* ov, jg_cde* cde, const uint32_t* item_slots, const float* d_pose,
+                                    const float* rad, uint64_t* n_pairs_out, void* stream);
+/* The overlap's current candidate pairs copied to the host ([n][2]); with pairs_out NULL only
+ * their number. */
+jg_status jg_overlap_get_pairs(const jg_overlap* ov, uint32_t* pairs_out, uint64_t cap, uint64_t* n_out);
 /* Candidate pairs for the overlap quantity: items whose bounding circles (centre (x, y),
  * radius rad) overlap, found with a uniform grid over `bbox` of 2^depth cells per side (the
  * leaf cells of a quadtree of that depth).  Writes up to `cap` pairs (a < b) and their count. */
 jg_status jg_overlap_candidate_pairs(const float* pose, const float* rad, uint32_t n_items, const float bbox[4],
                                      uint32_t depth, uint32_t* pairs_out, uint64_t cap, uint64_t* n_out);
+
+/* ---------------------------------------------------------------- graphs */
+
+/* Captures the launches the calls between jg_graph_begin and jg_graph_end make on `stream`
+ * (not the null stream) into one HIP graph, replayed by jg_graph_launch: for instance a
+ * step of jg_plan_detect, jg_plan_collect and jg_overlap_fwd / jg_overlap_bwd.  No reference
+ * counterpart.  Only launches may happen while capturing: call each captured entry point once
+ * beforehand (it builds its tables and buffers then), and update no engine the captured calls
+ * use while the graph is in use.  A call that would allocate or copy fails the capture, and
+ * jg_graph_end reports the error. */
+typedef struct jg_graph jg_graph;
+jg_status jg_graph_begin(void* stream);
+jg_status jg_graph_end(void* stream, jg_graph** out);
+jg_status jg_graph_launch(const jg_graph* graph, void* stream);
+void jg_graph_free(jg_graph* graph);
 
 /* Version / build info. */
 const char* jg_version(void);

@@ -599,6 +599,47 @@ void collect_overflow_launch(const SetDesc* sets, const uint32_t* counters, cons
 }
 }  // namespace jg
 
+// ------------------------------------------------------------------------------ ABI: graphs
+struct jg_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~jg_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+extern "C" jg_status jg_graph_begin(void* stream) {
+    if (!stream) return fail(JG_ERR_INVALID_ARG, "capture needs a stream other than the null stream");
+    jg_status s = device_ready();
+    if (s != JG_OK) return s;
+    return hip_check(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal),
+                     "hipStreamBeginCapture");
+}
+
+extern "C" jg_status jg_graph_end(void* stream, jg_graph** out) {
+    if (!stream || !out) return fail(JG_ERR_INVALID_ARG, "null argument");
+    hipGraph_t g = nullptr;
+    jg_status s = hip_check(hipStreamEndCapture((hipStream_t)stream, &g), "hipStreamEndCapture");
+    if (s != JG_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return s;
+    }
+    auto r = std::make_unique<jg_graph>();
+    r->graph = g;
+    s = hip_check(hipGraphInstantiate(&r->exec, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+    if (s != JG_OK) return s;
+    *out = r.release();
+    return JG_OK;
+}
+
+extern "C" jg_status jg_graph_launch(const jg_graph* g, void* stream) {
+    if (!g || !g->exec) return fail(JG_ERR_INVALID_ARG, "null graph");
+    return hip_check(hipGraphLaunch(g->exec, (hipStream_t)stream), "hipGraphLaunch");
+}
+
+extern "C" void jg_graph_free(jg_graph* g) { delete g; }
+
 // ------------------------------------------------------------------------------ ABI: polygons
 using namespace jg;
 

@@ -45,14 +45,15 @@ using namespace jg;
 
 namespace {
 
-constexpr uint32_t CP = 16;              // queries per collection pass
+constexpr uint32_t CP = 8;              // queries per collection pass
 constexpr uint32_t CG = WAVE / CP;       // lanes per query in the set-up, containment and result phases
 constexpr uint32_t SET_CAP = 16;         // slots per query collector in LDS
-constexpr uint32_t NC = 16;              // candidates per query (bits of an edge's mask)
-constexpr uint32_t CE_CAP = 256;         // edge-test queue entries
-constexpr uint32_t CE_LIMIT = 256;       // edge tests one visit pushes: fewer (also below the scan's 16 bits)
+constexpr uint32_t NC = 8;              // candidates per query (bits of an edge's mask)
+constexpr uint32_t CE_CAP = 128;         // edge-test queue entries
+constexpr uint32_t CE_LIMIT = 128;       // edge tests one visit pushes: fewer (also below the scan's 16 bits)
 static_assert(64u * (CE_LIMIT - 1u) < 65536u, "a wave's edge pushes must fit 16 bits");
-constexpr uint32_t PRUNE_MAX = 8;        // surrogate visits: nodes with at most this many entries are checked
+constexpr bool INLINE_EDGES = false;   // test a visit's hazard edges in the visiting lane (no edge queue)
+constexpr uint32_t PRUNE_MAX = 4;        // surrogate visits: nodes with at most this many entries are checked
 constexpr uint32_t SURR_MASK = 0u;       // emask of a pole or pier: the node-entry check applies
 constexpr uint32_t C_EXT = 0x80000000u;  // candidate record flags: exterior hazard, Entire in the virtual root
 constexpr uint32_t C_ENTIRE = 0x40000000u;
@@ -71,6 +72,9 @@ struct CollectLds {
     uint32_t cdone[CP];                  // bit i: candidate i is in the set
     uint32_t ovf[CP];                    // per query: its set outgrew SET_CAP
     int32_t vroot[CP];
+    float4 qbox[CP];                     // per query: its transformed bbox
+    uint32_t vbeg[CP], vcnt[CP];         // its virtual root's entry range
+    uint32_t vpre[CP + 1];               // prefix sums of vcnt over the pass
     uint8_t edge_ent[CE_CAP];
     uint8_t ent_slot[WL_ENT];            // query (0..CP-1) | WL_CIRCLE
 };
@@ -241,6 +245,14 @@ __device__ __forceinline__ void wave_collect(const CdeView& c, CollectLds& L, ui
                     } else {
                         m = edge_quadrants(g, nd.bbox);
                     }
+                } else if (INLINE_EDGES) {
+                    // the visit inserts its Entire hazards and tests its Partial hazards' edges itself
+                    for (uint32_t i = nd.haz_begin; i < nd.haz_begin + cnt; i++) {
+                        const uint32_t sf = c.ent[i].slot_flags;
+                        const uint32_t sl = sf & SLOT_MASK;
+                        if ((sf & ENTIRE_BIT) && !set_has(L, qs, sl)) set_put(L, qs, sl);
+                    }
+                    test_inline(c, L, nd, qs, circle, g);
                 } else {
                     for (uint32_t i = nd.haz_begin; i < nd.haz_begin + cnt; i++) {
                         const DevEntry e = c.ent[i];
@@ -290,13 +302,6 @@ __device__ __forceinline__ void wave_collect(const CdeView& c, CollectLds& L, ui
         if (fits) ne += e_tot;
     }
     wave_sync();
-}
-
-__device__ __forceinline__ uint32_t push_root(CollectLds& L, uint32_t n, bool want, uint32_t ent, uint32_t node,
-                                              uint32_t lane) {
-    const uint64_t b = __ballot(want);
-    if (want) L.nodes[n + lanes_below(b, lane)] = (ent << 24) | node;
-    return n + (uint32_t)__builtin_popcountll(b);
 }
 
 __device__ __forceinline__ uint64_t group_sum_u64(uint64_t v) {
@@ -363,69 +368,92 @@ __device__ __forceinline__ void collect_pass(const SetDesc& sd, uint32_t set_ind
         L.cdone[qs] = 0u;
     }
     int vr = 0;
-    if (active) vr = virtual_root(c, bb);
-    wave_sync();
-    // the candidates: the virtual root's entries whose widened box meets the query's bbox, and
-    // the exterior unless the bbox is inside the inner box; four entries in flight per lane
     uint32_t vcnt = 0, ve0 = 0;
     if (active) {
+        vr = virtual_root(c, bb);
         const DevNode vn = c.nodes[vr];
         ve0 = vn.haz_begin;
         vcnt = vn.haz_info & COUNT_MASK;
+    }
+    if (sub == 0) {
+        L.vroot[qs] = vr;
+        L.qbox[qs] = bb;
+        L.vbeg[qs] = ve0;
+        L.vcnt[qs] = vcnt;
+    }
+    wave_sync();
+    // the candidates: the virtual root's entries whose widened box meets the query's bbox, and
+    // the exterior unless the bbox is inside the inner box.  The pass's (query, entry) pairs are
+    // scanned by the whole wave, four per lane in flight, so that one query of a shallow virtual
+    // root (hundreds of entries) does not hold the other queries' lanes
+    if (lane <= CP) {
+        uint32_t pre = 0;
+        for (uint32_t j = 0; j < lane; j++) pre += L.vcnt[j];
+        L.vpre[lane] = pre;
+    }
+    wave_sync();
+    {
+        const uint32_t T = L.vpre[CP];
         const float4 inner = inner_shrunk(c);
-        for (uint32_t k0 = 0; k0 < vcnt; k0 += 4u * CG) {
-            uint32_t sf[4];
+        for (uint32_t p0 = 0; p0 < T; p0 += 4u * WAVE) {
+            uint32_t sf[4], qq[4];
 #pragma unroll
             for (uint32_t r = 0; r < 4; r++) {
-                const uint32_t k = k0 + r * CG + sub;
-                sf[r] = c.ent[ve0 + (k < vcnt ? k : 0u)].slot_flags;
+                const uint32_t p = p0 + r * WAVE + lane;
+                uint32_t lo = 0;                  // the query of pair p: last lo with vpre[lo] <= p
+                for (uint32_t step = CP / 2; step > 0; step >>= 1)
+                    if (L.vpre[lo + step] <= p) lo += step;
+                qq[r] = lo;
+                sf[r] = p < T ? c.ent[L.vbeg[lo] + (p - L.vpre[lo])].slot_flags : 0u;
             }
             float4 hb[4];
 #pragma unroll
             for (uint32_t r = 0; r < 4; r++) hb[r] = c.haz_bbox[sf[r] & SLOT_MASK];
 #pragma unroll
             for (uint32_t r = 0; r < 4; r++) {
-                if (k0 + r * CG + sub >= vcnt) continue;
+                if (p0 + r * WAVE + lane >= T) continue;
+                const uint32_t q = qq[r];
+                const float4 qb = L.qbox[q];
                 const uint32_t sl = sf[r] & SLOT_MASK;
                 const bool ext = sl == c.exterior_slot;
                 const float4 wb = ext ? inner : widened(hb[r]);
-                if (ext ? box_inside(bb, inner) : !boxes_meet(bb, wb)) continue;
-                const uint32_t at = atomicAdd(&L.ncand[qs], 1u);
+                if (ext ? box_inside(qb, inner) : !boxes_meet(qb, wb)) continue;
+                const uint32_t at = atomicAdd(&L.ncand[q], 1u);
                 if (at < NC) {
-                    L.cand[qs * NC + at] = sl | (ext ? C_EXT : 0u) | ((sf[r] & ENTIRE_BIT) ? C_ENTIRE : 0u);
-                    L.cbox[qs * NC + at] = wb;
+                    L.cand[q * NC + at] = sl | (ext ? C_EXT : 0u) | ((sf[r] & ENTIRE_BIT) ? C_ENTIRE : 0u);
+                    L.cbox[q * NC + at] = wb;
                 }
             }
         }
     }
     wave_sync();
     if (active) {
-        if (sub == 0) L.vroot[qs] = vr;
         // a pre-seeded collector, then the exterior when the shape leaves the engine's bbox
         // (cd_engine.rs:262-264); the traversal goes on regardless
         for (uint32_t i = sub; i < sd.n_seed; i += CG) set_put(L, qs, sd.seed[i]);
         if (sub == 0 && box_relation(c.nodes[0].bbox, bb) != REL_SURROUNDING) set_put(L, qs, c.exterior_slot);
     }
-    // every query's ff poles / piers from the root and edges from its virtual root, WL_ENT
-    // entities per traversal
+    // every query's ff poles / piers from the root and edges from its virtual root.  A cursor
+    // walks the entities 64 at a time; those that may add to their query's set take the next
+    // free places of the traversal (WL_ENT), which runs when they are full or the entities end
     const uint32_t n_ff = it.n_poles + it.n_piers;
     const uint32_t n_ent = n_ff + it.n_v;
     const uint32_t total = P * n_ent;
-    for (uint32_t e0 = 0; e0 < total; e0 += WL_ENT) {
+    uint32_t cursor = 0;
+    while (cursor < total) {
         wave_sync();
         uint32_t n = 0;
-        for (uint32_t j = 0; j < WL_ENT; j += WAVE) {
-            const uint32_t o = j + lane;
-            const uint32_t e = e0 + o;
+        while (cursor < total && n < WL_ENT) {
+            const uint32_t e = cursor + lane;
             bool want = false;
-            uint32_t root = 0;
+            uint32_t root = 0, em = SURR_MASK, qi = 0;
+            bool circle = false;
+            float4 gm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (e < total) {
-                const uint32_t qi = e / n_ent, k = e - qi * n_ent;
+                qi = e / n_ent;
+                const uint32_t k = e - qi * n_ent;
                 if (!L.ovf[qi]) {
                     const Mat mq = load_mat(m6, sd.first + q0 + qi);
-                    bool circle = false;
-                    float4 gm;
-                    uint32_t em = SURR_MASK;
                     if (k < n_ff) {
                         const float4 p = it.ff[k];
                         circle = k < it.n_poles;
@@ -436,13 +464,15 @@ __device__ __forceinline__ void collect_pass(const SetDesc& sd, uint32_t set_ind
                             const float2 a = xform(mq, p.x, p.y), b = xform(mq, p.z, p.w);
                             gm = make_float4(a.x, a.y, b.x, b.y);
                         }
+                        want = true;
                     } else {
                         const uint32_t i = k - n_ff, i1 = i + 1 == it.n_v ? 0 : i + 1;
                         const float2 a = xform(mq, it.v[i].x, it.v[i].y), b = xform(mq, it.v[i1].x, it.v[i1].y);
                         gm = make_float4(a.x, a.y, b.x, b.y);
                         root = (uint32_t)L.vroot[qi];
                         // the candidates this edge's box may collect (all of them when the
-                        // query has more than NC)
+                        // query has more than NC); an edge whose candidates are all in the
+                        // set already adds nothing
                         const uint32_t nc = L.ncand[qi];
                         if (nc > NC) {
                             em = 0xffffffffu;
@@ -456,18 +486,32 @@ __device__ __forceinline__ void collect_pass(const SetDesc& sd, uint32_t set_ind
                                 em |= may ? (1u << t) : 0u;
                             }
                         }
-                        // an edge that may collect nothing is not pushed
-                        want = em != 0u;
-                    }
-                    if (k < n_ff) want = true;
-                    if (want) {
-                        L.geom[o] = gm;
-                        L.emask[o] = em;
-                        L.ent_slot[o] = (uint8_t)(qi | (circle ? WL_CIRCLE : 0u));
+                        want = (em & ~L.cdone[qi]) != 0u;
                     }
                 }
             }
-            n = push_root(L, n, want, o, root, lane);
+            // the first WL_ENT - n wanted entities of these 64 take places n, n + 1, ...
+            const uint64_t wb = __ballot(want);
+            const uint32_t rank = lanes_below(wb, lane);
+            const uint32_t room = WL_ENT - n;
+            const uint32_t n_want = (uint32_t)__builtin_popcountll(wb);
+            if (want && rank < room) {
+                const uint32_t o = n + rank;
+                L.geom[o] = gm;
+                L.emask[o] = em;
+                L.ent_slot[o] = (uint8_t)(qi | (circle ? WL_CIRCLE : 0u));
+                L.nodes[o] = (o << 24) | root;
+            }
+            if (n_want <= room) {
+                n += n_want;
+                cursor += WAVE;
+            } else {
+                // resume at the first wanted entity that found no place
+                uint64_t r = wb;
+                for (uint32_t t = 0; t < room; t++) r &= r - 1ull;
+                cursor += (uint32_t)__builtin_ctzll(r);
+                n = WL_ENT;
+            }
         }
         wave_collect(c, L, n, lane);
     }

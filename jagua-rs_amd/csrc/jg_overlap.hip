@@ -1,7 +1,8 @@
 // Pole-pair overlap quantity and its gradient with respect to every item's (x, y, theta).
 //
 // Forward:  one wavefront per candidate item pair (lanes = poles of item a, item b's
-//           rotated poles staged in LDS) and one per item for the container edges; each
+//           rotated poles staged in LDS) and one per item for the container term (each
+//           pole's signed distance to the container boundary); each
 //           wave writes one partial; a two-pass fixed-order reduction (f64 accumulation)
 //           produces Q.  Penetrations are those of Circle::separation_distance (circle.rs:
 //           184-200, (r1 + r2) - d) and of the circle-edge distance (circle.rs:216-229).
@@ -51,6 +52,28 @@ __device__ __forceinline__ float2 edge_offset(float4 e, float px, float py) {
     if (t > 1.0f) return make_float2(px - e.z, py - e.w);
     const float w = (a * cy - b * cx) / len_sq;
     return make_float2(w * cy, -(w * cx));
+}
+
+// Signed distance from point (px, py) to the container boundary (the v.n_cedges edges,
+// translated by -t): the distance to the nearest edge, positive when the point is inside by
+// ray-crossing parity, and the offset from that edge's closest point to the point.
+__device__ __forceinline__ float container_distance(const float4* cedges, uint32_t n, float tx, float ty, float px,
+                                                    float py, float2& off) {
+    float best = INFINITY;
+    off = make_float2(0.0f, 0.0f);
+    uint32_t crossings = 0;
+    for (uint32_t e = 0; e < n; e++) {
+        const float4 ew = cedges[e];
+        const float4 ed = make_float4(ew.x - tx, ew.y - ty, ew.z - tx, ew.w - ty);
+        const float2 o = edge_offset(ed, px, py);
+        const float d = sqrtf(o.x * o.x + o.y * o.y);
+        if (d < best) {
+            best = d;
+            off = o;
+        }
+        if ((ed.y > py) != (ed.w > py) && px < ed.x + (py - ed.y) * (ed.z - ed.x) / (ed.w - ed.y)) crossings++;
+    }
+    return (crossings & 1u) ? best : -best;
 }
 
 struct OvView {
@@ -109,6 +132,7 @@ __global__ __launch_bounds__(64 * OV_WAVES) void k_ov_fwd_pairs(OvView v, const 
     if (lane == 0) partial[pid] = acc;
 }
 
+// the container term: per pole, (r - D)^2 when r > D, D its signed distance to the container
 __global__ __launch_bounds__(64 * OV_WAVES) void k_ov_fwd_container(OvView v, const float* __restrict__ pose3,
                                                           float* __restrict__ partial) {
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
@@ -119,21 +143,15 @@ __global__ __launch_bounds__(64 * OV_WAVES) void k_ov_fwd_container(OvView v, co
     sincosf(pa.z, &s, &c);
     const uint32_t a0 = v.off[item], na = v.off[item + 1] - a0;
     float acc = 0.0f;
-    for (uint32_t i = lane; i < na; i += 64) {
-        const float4 p = v.poles[a0 + i];
-        const float2 rp = rot(s, c, p);
-        for (uint32_t e = 0; e < v.n_cedges; e++) {
-            const float4 ew = v.cedges[e];
-            const float4 ed = make_float4(ew.x - pa.x, ew.y - pa.y, ew.z - pa.x, ew.w - pa.y);
-            const float2 o = edge_offset(ed, rp.x, rp.y);
-            const float dx = o.x, dy = o.y;
-            const float d = sqrtf(dx * dx + dy * dy);
-            if (d < p.z) {
-                const float pen = p.z - d;
-                acc += pen * pen;
-            }
+    if (v.n_cedges > 0)
+        for (uint32_t i = lane; i < na; i += 64) {
+            const float4 p = v.poles[a0 + i];
+            const float2 rp = rot(s, c, p);
+            float2 o;
+            const float D = container_distance(v.cedges, v.n_cedges, pa.x, pa.y, rp.x, rp.y, o);
+            const float pen = p.z - D;
+            if (pen > 0.0f) acc += pen * pen;
         }
-    }
     acc = wsum(acc);
     if (lane == 0) partial[item] = acc;
 }
@@ -238,26 +256,23 @@ __global__ __launch_bounds__(64 * OV_WAVES) void k_ov_bwd_container(OvView v, co
     sincosf(pa.z, &s, &c);
     const uint32_t a0 = v.off[item], na = v.off[item + 1] - a0;
     float gx = 0, gy = 0, gt = 0;
-    for (uint32_t i = lane; i < na; i += 64) {
-        const float4 p = v.poles[a0 + i];
-        const float2 rp = rot(s, c, p);
-        float gcx = 0.0f, gcy = 0.0f;
-        for (uint32_t e = 0; e < v.n_cedges; e++) {
-            const float4 ew = v.cedges[e];
-            const float4 ed = make_float4(ew.x - pa.x, ew.y - pa.y, ew.z - pa.x, ew.w - pa.y);
-            const float2 o = edge_offset(ed, rp.x, rp.y);
-            const float dx = o.x, dy = o.y;
-            const float d = sqrtf(dx * dx + dy * dy);
-            if (d < p.z && d > 0.0f) {
-                const float k = -2.0f * (p.z - d) / d;
-                gcx += k * dx;
-                gcy += k * dy;
+    if (v.n_cedges > 0)
+        for (uint32_t i = lane; i < na; i += 64) {
+            const float4 p = v.poles[a0 + i];
+            const float2 rp = rot(s, c, p);
+            float2 o;
+            const float D = container_distance(v.cedges, v.n_cedges, pa.x, pa.y, rp.x, rp.y, o);
+            const float pen = p.z - D;
+            const float d = fabsf(D);
+            if (pen > 0.0f && d > 0.0f) {
+                // dQ/dc = 2 pen (-dD/dc), dD/dc = sign(D) o / d
+                const float k = -2.0f * pen * (D > 0.0f ? 1.0f : -1.0f) / d;
+                const float gcx = k * o.x, gcy = k * o.y;
+                gx += gcx;
+                gy += gcy;
+                gt += -gcx * rp.y + gcy * rp.x;
             }
         }
-        gx += gcx;
-        gy += gcy;
-        gt += -gcx * rp.y + gcy * rp.x;
-    }
     gx = wsum(gx); gy = wsum(gy); gt = wsum(gt);
     if (lane == 0) {
         out3[3 * item] = gx; out3[3 * item + 1] = gy; out3[3 * item + 2] = gt;
@@ -281,6 +296,183 @@ __global__ __launch_bounds__(256) void k_ov_bwd_gather(uint32_t n_items, const u
     grad[3 * item + 2] = (float)gt;
 }
 
+// ---- candidate pairs from the engine's quadtree --------------------------------------
+// One wave per item: the quadtree nodes whose box meets the box of the item's bounding circle
+// are visited breadth first (a frontier in LDS; when a level outgrows it, the lanes walk the
+// rest of their subtrees depth first), every hazard slot of the leaves reached is marked in an
+// LDS bitmask, and the marked slots that are items k > i with an overlapping bounding circle
+// are the item's pairs (i, k), counted (fill = 0) or written in slot order from off_a[i]
+// (fill = 1).  The order depends on nothing but the tree, so the pairs are deterministic.
+namespace {
+constexpr uint32_t OVP_FCAP = 512;
+
+__device__ __forceinline__ bool box_meets(float4 a, float4 b) {
+    return a.x <= b.z && a.z >= b.x && a.y <= b.w && a.w >= b.y;
+}
+}  // namespace
+
+__global__ __launch_bounds__(64) void k_ovp_items(CdeView c, const uint32_t* __restrict__ slot_item,
+                                                  const float* __restrict__ pose3, const float* __restrict__ rad,
+                                                  uint32_t n_items, uint32_t words, uint32_t fill,
+                                                  uint32_t* __restrict__ cnt_a, const uint32_t* __restrict__ off_a,
+                                                  uint2* __restrict__ pairs) {
+    extern __shared__ uint32_t bits[];
+    __shared__ uint32_t fr[2][OVP_FCAP];
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t w = lane; w < words; w += 64) bits[w] = 0u;
+    __syncthreads();
+    for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
+        const float x = pose3[3 * i], y = pose3[3 * i + 1], r = rad[i];
+        const float4 cb = make_float4(x - r, y - r, x + r, y + r);
+        // the leaves' slots into the bitmask
+        auto visit = [&](int n, int& fc) -> uint32_t {
+            const DevNode nd = c.nodes[n];
+            if (nd.first_child < 0) {
+                const uint32_t cnt = nd.haz_info & COUNT_MASK;
+                for (uint32_t k = 0; k < cnt; k++) {
+                    const uint32_t sl = c.ent[nd.haz_begin + k].slot_flags & SLOT_MASK;
+                    atomicOr(&bits[sl >> 5], 1u << (sl & 31u));
+                }
+                return 0u;
+            }
+            fc = nd.first_child;
+            uint32_t m = 0;
+            for (int k = 0; k < 4; k++) m |= box_meets(box_quadrant(nd.bbox, k), cb) ? (1u << k) : 0u;
+            return m;
+        };
+        uint32_t nf = box_meets(c.nodes[0].bbox, cb) ? 1u : 0u, cur = 0;
+        if (lane == 0) fr[0][0] = 0u;
+        __syncthreads();
+        while (nf > 0u) {
+            uint32_t nb = 0;
+            for (uint32_t b0 = 0; b0 < nf; b0 += 64u) {
+                const bool mine = b0 + lane < nf;
+                int fc = -1;
+                uint32_t m = 0;
+                int n = 0;
+                if (mine) {
+                    n = (int)fr[cur][b0 + lane];
+                    m = visit(n, fc);
+                }
+                const uint32_t k = (uint32_t)__builtin_popcount(m);
+                const uint32_t incl = wave_incl_scan(k, lane);
+                const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+                if (nb + tot <= OVP_FCAP) {
+                    uint32_t at = nb + incl - k;
+                    for (uint32_t b = m; b; b &= b - 1u) fr[cur ^ 1u][at++] = (uint32_t)(fc + __builtin_ctz(b));
+                    nb += tot;
+                } else {
+                    // the next level outgrows the frontier: this lane walks its node's subtrees
+                    for (uint32_t b = m; b; b &= b - 1u) (void)descend(c, fc + __builtin_ctz(b), visit);
+                }
+            }
+            __syncthreads();
+            cur ^= 1u;
+            nf = nb;
+        }
+        __syncthreads();
+        // the marked slots, in slot order: items k > i whose bounding circles overlap item i's
+        uint32_t base = fill ? off_a[i] : 0u;
+        uint32_t count = 0;
+        for (uint32_t w0 = 0; w0 < words; w0 += 64u) {
+            const uint32_t w = w0 + lane;
+            const uint32_t word = w < words ? bits[w] : 0u;
+            // a neighbour: an item k > i whose bounding circle overlaps item i's
+            auto neighbour = [&](uint32_t b, uint32_t& k) {
+                k = slot_item[w * 32u + b];
+                if (k == 0xffffffffu || k <= i) return false;
+                const float dx = pose3[3 * k] - x, dy = pose3[3 * k + 1] - y, rr = rad[k] + r;
+                return dx * dx + dy * dy <= rr * rr;
+            };
+            uint32_t nm = 0;
+            for (uint32_t wb = word; wb; wb &= wb - 1u) {
+                uint32_t k;
+                nm += neighbour((uint32_t)__builtin_ctz(wb), k) ? 1u : 0u;
+            }
+            const uint32_t incl = wave_incl_scan(nm, lane);
+            const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+            if (fill) {
+                uint32_t at = base + incl - nm;
+                for (uint32_t wb = word; wb; wb &= wb - 1u) {
+                    uint32_t k;
+                    if (neighbour((uint32_t)__builtin_ctz(wb), k)) pairs[at++] = make_uint2(i, k);
+                }
+            }
+            base += tot;
+            count += tot;
+            if (w < words) bits[w] = 0u;
+        }
+        if (!fill && lane == 0) cnt_a[i] = count;
+        __syncthreads();
+    }
+}
+
+// Exclusive prefix sums of in[0..n) into out[0..n], out[n] the total: one workgroup of 1024.
+__global__ __launch_bounds__(1024) void k_ovp_scan(const uint32_t* __restrict__ in, uint32_t n,
+                                                   uint32_t* __restrict__ out) {
+    __shared__ uint32_t sh[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (n + 1023u) / 1024u;
+    const uint32_t lo = min(n, t * per), hi = min(n, lo + per);
+    uint32_t s = 0;
+    for (uint32_t k = lo; k < hi; k++) s += in[k];
+    sh[t] = s;
+    __syncthreads();
+    for (uint32_t o = 1; o < 1024u; o <<= 1) {
+        const uint32_t v = t >= o ? sh[t - o] : 0u;
+        __syncthreads();
+        sh[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = sh[t] - s;
+    for (uint32_t k = lo; k < hi; k++) {
+        out[k] = run;
+        run += in[k];
+    }
+    if (t == 1023u) out[n] = sh[1023];
+}
+
+__global__ __launch_bounds__(256) void k_ovp_zero(uint32_t* __restrict__ x, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_ovp_count_b(const uint2* __restrict__ pairs, uint32_t n_pairs,
+                                                     uint32_t* __restrict__ cnt_b) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n_pairs) atomicAdd(&cnt_b[pairs[p].y], 1u);
+}
+
+__global__ __launch_bounds__(256) void k_ovp_fill_b(const uint2* __restrict__ pairs, uint32_t n_pairs,
+                                                    uint32_t* __restrict__ cur_b, uint32_t* __restrict__ adj_b) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n_pairs) adj_b[atomicAdd(&cur_b[pairs[p].y], 1u)] = p;
+}
+
+// Per item: its adjacency in a fixed order, the pairs where it is b (ascending, sorted here
+// from the order the atomics left), then those where it is a.
+__global__ __launch_bounds__(256) void k_ovp_adj(uint32_t n_items, const uint32_t* __restrict__ off_a,
+                                                 const uint32_t* __restrict__ off_b, uint32_t* __restrict__ adj_b,
+                                                 uint32_t* __restrict__ adj_off, uint32_t* __restrict__ adj) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n_items) return;
+    adj_off[i] = off_a[i] + off_b[i];
+    if (i == n_items) return;
+    const uint32_t b0 = off_b[i], b1 = off_b[i + 1];
+    for (uint32_t k = b0 + 1; k < b1; k++) {          // insertion sort (a few entries)
+        const uint32_t v = adj_b[k];
+        uint32_t j = k;
+        while (j > b0 && adj_b[j - 1] > v) {
+            adj_b[j] = adj_b[j - 1];
+            j--;
+        }
+        adj_b[j] = v;
+    }
+    uint32_t at = off_a[i] + off_b[i];
+    for (uint32_t k = b0; k < b1; k++) adj[at++] = (adj_b[k] << 1) | 1u;
+    for (uint32_t p = off_a[i]; p < off_a[i + 1]; p++) adj[at++] = p << 1;
+}
+
 // ---------------------------------------------------------------------------------------
 struct jg_overlap {
     uint32_t n_items = 0, n_cedges = 0;
@@ -288,6 +480,7 @@ struct jg_overlap {
     DevBlob geo;                  // poles, offsets, container edges
     DevBlob pairs;                // pairs + CSR adjacency
     DevBlob scratch;              // partials, per-pair gradients, block sums
+    DevBlob work;                 // pair search: slot -> item, radii, counts and offsets
     OvView view{};
     const uint32_t* adj_off = nullptr;
     const uint32_t* adj = nullptr;
@@ -394,6 +587,97 @@ extern "C" jg_status jg_overlap_set_pairs(jg_overlap* ov, const uint32_t* pairs,
     jg_status s = hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     if (s != JG_OK) return s;
     return ov_upload_pairs(ov, pairs, n_pairs);
+}
+
+// Candidate pairs from the engine's quadtree (k_ovp_*), then the CSR adjacency of the
+// backward gather, all on the device; the pair count is read back once.
+extern "C" jg_status jg_overlap_pairs_from_cde(jg_overlap* ov, jg_cde* cde, const uint32_t* item_slots,
+                                               const float* d_pose, const float* rad, uint64_t* n_pairs_out,
+                                               void* stream) {
+    if (!ov || !cde || !d_pose || (ov->n_items && (!item_slots || !rad))) return fail(JG_ERR_INVALID_ARG, "null argument");
+    jg_status s = cde_upload(cde);
+    if (s != JG_OK) return s;
+    const uint32_t n = ov->n_items;
+    const uint32_t n_slots = cde->view.n_slots;
+    const uint32_t words = (n_slots + 31u) / 32u;
+    if ((size_t)words * 4 > 60000) return fail(JG_ERR_CAPACITY, "too many hazard slots for the LDS bitmask");
+    std::vector<uint32_t> slot_item(n_slots, 0xffffffffu);
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t sl = item_slots[i];
+        if (sl >= n_slots || !cde->map.slots[sl].occ) return fail(JG_ERR_INVALID_ARG, "item slot is not a hazard of the engine");
+        slot_item[sl] = i;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    // work arrays: slot_item, rad, cnt_a, off_a, cnt_b, off_b, cur_b
+    const size_t o_si = 0, o_rad = al(4 * (size_t)n_slots), o_ca = o_rad + al(4 * (size_t)n + 4);
+    const size_t o_oa = o_ca + al(4 * (size_t)n + 4), o_cb = o_oa + al(4 * (size_t)n + 4);
+    const size_t o_ob = o_cb + al(4 * (size_t)n + 4), o_cu = o_ob + al(4 * (size_t)n + 4);
+    const size_t work = o_cu + al(4 * (size_t)n + 4);
+    s = hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");   // launches may still read the work buffer
+    if (s != JG_OK) return s;
+    s = ov->work.reserve(work);
+    if (s != JG_OK) return s;
+    uint8_t* w = (uint8_t*)ov->work.ptr;
+    uint32_t* d_si = (uint32_t*)(w + o_si);
+    float* d_rad = (float*)(w + o_rad);
+    uint32_t *ca = (uint32_t*)(w + o_ca), *oa = (uint32_t*)(w + o_oa), *cb = (uint32_t*)(w + o_cb);
+    uint32_t *ob = (uint32_t*)(w + o_ob), *cu = (uint32_t*)(w + o_cu);
+    s = hip_check(hipMemcpy(d_si, slot_item.data(), 4 * (size_t)n_slots, hipMemcpyHostToDevice), "hipMemcpy slots");
+    if (s == JG_OK) s = hip_check(hipMemcpy(d_rad, rad, 4 * (size_t)n, hipMemcpyHostToDevice), "hipMemcpy radii");
+    if (s != JG_OK) return s;
+    const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(n, 4096u));
+    const size_t lds = (size_t)words * 4;
+    hipLaunchKernelGGL(k_ovp_items, dim3(grid), dim3(64), lds, st, cde->view, d_si, d_pose, d_rad, n, words, 0u, ca,
+                       oa, (uint2*)nullptr);
+    hipLaunchKernelGGL(k_ovp_scan, dim3(1), dim3(1024), 0, st, ca, n, oa);
+    uint32_t total = 0;
+    s = hip_check(hipMemcpyAsync(&total, oa + n, 4, hipMemcpyDeviceToHost, st), "hipMemcpy pair count");
+    if (s == JG_OK) s = hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (s != JG_OK) return s;
+    // pairs, adjacency offsets, adjacency, the b-role lists
+    const size_t o_pairs = 0, o_off = al(8 * (size_t)total + 8), o_adj = o_off + al(4 * ((size_t)n + 1));
+    const size_t o_adjb = o_adj + al(8 * (size_t)total + 4);
+    const size_t need = o_adjb + al(4 * (size_t)total + 4);
+    s = ov->pairs.reserve(need);
+    if (s != JG_OK) return s;
+    uint8_t* b = (uint8_t*)ov->pairs.ptr;
+    uint2* pairs = (uint2*)(b + o_pairs);
+    uint32_t* adj_off = (uint32_t*)(b + o_off);
+    uint32_t* adj = (uint32_t*)(b + o_adj);
+    uint32_t* adj_b = (uint32_t*)(b + o_adjb);
+    hipLaunchKernelGGL(k_ovp_items, dim3(grid), dim3(64), lds, st, cde->view, d_si, d_pose, d_rad, n, words, 1u, ca,
+                       (const uint32_t*)oa, pairs);
+    hipLaunchKernelGGL(k_ovp_zero, dim3((n + 256) / 256), dim3(256), 0, st, cb, n + 1);
+    const uint32_t pb = (total + 255u) / 256u;
+    if (total) hipLaunchKernelGGL(k_ovp_count_b, dim3(pb), dim3(256), 0, st, (const uint2*)pairs, total, cb);
+    hipLaunchKernelGGL(k_ovp_scan, dim3(1), dim3(1024), 0, st, cb, n, ob);
+    s = hip_check(hipMemcpyAsync(cu, ob, 4 * (size_t)n, hipMemcpyDeviceToDevice, st), "hipMemcpy cursors");
+    if (s != JG_OK) return s;
+    if (total) hipLaunchKernelGGL(k_ovp_fill_b, dim3(pb), dim3(256), 0, st, (const uint2*)pairs, total, cu, adj_b);
+    hipLaunchKernelGGL(k_ovp_adj, dim3((n + 1 + 255) / 256), dim3(256), 0, st, n, (const uint32_t*)oa,
+                       (const uint32_t*)ob, adj_b, adj_off, adj);
+    s = hip_check(hipGetLastError(), "pair launches");
+    if (s == JG_OK) s = hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (s != JG_OK) return s;
+    ov->view.pairs = pairs;
+    ov->adj_off = adj_off;
+    ov->adj = adj;
+    ov->n_pairs = total;
+    ov->view.n_pairs = total;
+    if (n_pairs_out) *n_pairs_out = total;
+    const size_t scr = al(4 * (total + (size_t)n) + 4) + al(24 * (size_t)total + 24) + al(12 * (size_t)n + 12) +
+                       al(8 * 4096);
+    return ov->scratch.reserve(scr);
+}
+
+// The current candidate pairs, copied to the host ([n_pairs][2]).
+extern "C" jg_status jg_overlap_get_pairs(const jg_overlap* ov, uint32_t* pairs_out, uint64_t cap, uint64_t* n_out) {
+    if (!ov || !n_out) return fail(JG_ERR_INVALID_ARG, "null argument");
+    *n_out = ov->n_pairs;
+    if (!pairs_out) return JG_OK;
+    if (cap < ov->n_pairs) return fail(JG_ERR_CAPACITY, "pair buffer too small");
+    if (!ov->n_pairs) return JG_OK;
+    return hip_check(hipMemcpy(pairs_out, ov->view.pairs, 8 * ov->n_pairs, hipMemcpyDeviceToHost), "hipMemcpy pairs");
 }
 
 extern "C" void jg_overlap_free(jg_overlap* ov) { delete ov; }

@@ -473,6 +473,44 @@ def detect_sets_noalloc(sets, mats, out, chunk=0, workgroups=0, lanes_per_query=
     return out
 
 
+class Graph:
+    """The launches of the calls made inside `with Graph.capture(stream) as g:` on that stream
+    (a torch.cuda.Stream), captured once into a HIP graph (jg_graph_*) and replayed by
+    g.launch().  Call every captured operation once before capturing."""
+
+    def __init__(self):
+        self._h = C.c_void_p()
+        self._stream = None
+
+    @classmethod
+    def capture(cls, stream):
+        import contextlib
+
+        @contextlib.contextmanager
+        def cm():
+            g = cls()
+            g._stream = stream
+            check(N.lib().jg_graph_begin(_stream_ptr(stream)))
+            try:
+                yield g
+            except BaseException:
+                h = C.c_void_p()
+                N.lib().jg_graph_end(_stream_ptr(stream), C.byref(h))
+                if h.value:
+                    N.lib().jg_graph_free(h)
+                raise
+            check(N.lib().jg_graph_end(_stream_ptr(stream), C.byref(g._h)))
+        return cm()
+
+    def launch(self, stream=None):
+        check(N.lib().jg_graph_launch(self._h, _stream_ptr(stream if stream is not None else self._stream)))
+
+    def __del__(self, _N=N):
+        if getattr(self, "_h", None) and self._h.value and _N._lib is not None:
+            _N._lib.jg_graph_free(self._h)
+            self._h = C.c_void_p()
+
+
 class QueryPlan:
     """Batched persistent form: many (engine, item, transform range) sets in one launch.
 
@@ -632,6 +670,25 @@ class PoleOverlap:
     def set_pairs(self, pairs):
         pr = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
         check(N.lib().jg_overlap_set_pairs(self._h, _ptr(pr, C.c_uint32), len(pr)))
+
+    def pairs_from_cde(self, cde, item_keys, pose, radius, stream=None):
+        """Candidate pairs from the engine's quadtree on the GPU (jg_overlap_pairs_from_cde):
+        item k is the hazard item_keys[k] (HazKey) of cde, its bounding circle centred at
+        pose[k] (x, y) (torch GPU f32 [n, 3]) with radius radius[k].  Returns the pair count."""
+        slots = np.ascontiguousarray([k.idx for k in item_keys], dtype=np.uint32)
+        rad = _f32(radius).reshape(-1)
+        n = C.c_uint64()
+        check(N.lib().jg_overlap_pairs_from_cde(self._h, cde._h, _ptr(slots, C.c_uint32), C.c_void_p(pose.data_ptr()),
+                                                _ptr(rad), C.byref(n), _stream_ptr(stream)))
+        return n.value
+
+    def pairs(self):
+        """The current candidate pairs [n, 2] (host copy)."""
+        n = C.c_uint64()
+        check(N.lib().jg_overlap_get_pairs(self._h, None, 0, C.byref(n)))
+        out = np.empty((max(1, n.value), 2), dtype=np.uint32)
+        check(N.lib().jg_overlap_get_pairs(self._h, _ptr(out, C.c_uint32), n.value, C.byref(n)))
+        return out[: n.value]
 
     def forward(self, pose, out=None, stream=None):
         """Q for pose (torch GPU f32 [n_items, 3]); returns a GPU f32 tensor of shape [1]."""

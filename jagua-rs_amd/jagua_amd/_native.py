@@ -130,8 +130,14 @@ _SIGS = {
     "jg_overlap_fwd": (C.c_int32, [c_vp, c_vp, c_vp, c_vp]),
     "jg_overlap_bwd": (C.c_int32, [c_vp, c_vp, c_vp, c_vp]),
     "jg_overlap_free": (None, [c_vp]),
+    "jg_overlap_pairs_from_cde": (C.c_int32, [c_vp, c_vp, c_u32p, c_vp, c_f32p, c_u64p, c_vp]),
+    "jg_overlap_get_pairs": (C.c_int32, [c_vp, c_u32p, C.c_uint64, c_u64p]),
     "jg_overlap_candidate_pairs": (C.c_int32, [c_f32p, c_f32p, C.c_uint32, c_f32p, C.c_uint32, c_u32p, C.c_uint64,
                                                c_u64p]),
+    "jg_graph_begin": (C.c_int32, [c_vp]),
+    "jg_graph_end": (C.c_int32, [c_vp, C.POINTER(c_vp)]),
+    "jg_graph_launch": (C.c_int32, [c_vp, c_vp]),
+    "jg_graph_free": (None, [c_vp]),
     "jg_version": (C.c_char_p, []),
     "jg_last_error": (C.c_char_p, []),
 }

@@ -190,7 +190,19 @@ def config4_workload(seed=4, n_items=4096, width=2000.0, height=800.0, r_lo=6.0,
     pairs = candidate_pairs(pose, rad, [0.0, 0.0, width, height], depth)
     cedges = np.array([[0, 0, width, 0], [width, 0, width, height], [width, height, 0, height],
                        [0, height, 0, 0]], dtype=F32)
-    return dict(poles=allp, off=off, pose=pose, rad=rad, pairs=pairs, cedges=cedges)
+    return dict(poles=allp, off=off, pose=pose, rad=rad, pairs=pairs, cedges=cedges, polys=polys)
+
+
+def overlap_engine(w, depth=8, cd_threshold=16):
+    """A CDEngine holding the overlap workload's items as placed hazards at their poses
+    (DTransformation (theta, x, y) of pose (x, y, theta)) in its width x height rectangle, for
+    quadtree pair candidates.  Returns (engine, item keys)."""
+    ce = w["cedges"]
+    width, height = float(ce[:, [0, 2]].max()), float(ce[:, [1, 3]].max())
+    cont = np.array([[0, 0], [width, 0], [width, height], [0, height]], dtype=F32)
+    items = [(w["polys"][i], np.array([w["pose"][i, 2], w["pose"][i, 0], w["pose"][i, 1]], dtype=F32))
+             for i in range(len(w["polys"]))]
+    return build_cde(Scene(cont, [], items, depth, cd_threshold))
 
 
 def _log_uniform_int(rng, lo, hi):

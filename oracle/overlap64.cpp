@@ -53,11 +53,15 @@ void or_overlap64(const double* pose, const float* poles, const uint32_t* off, u
             }
         }
     }
-    for (uint32_t a = 0; a < n; a++)
+    // container term: the pole's signed distance D to the container boundary (the k edges,
+    // positive inside by ray-crossing parity), penetration r - D when positive
+    for (uint32_t a = 0; a < n && k > 0; a++)
         for (uint32_t i = off[a]; i < off[a + 1]; i++) {
             double cx, cy, rx, ry;
             world(a, i, cx, cy, rx, ry);
             const double r = poles[3 * i + 2];
+            double best = INFINITY, bx = 0.0, by = 0.0;
+            uint32_t crossings = 0;
             for (uint32_t e = 0; e < k; e++) {
                 const double x1 = cedges[4 * e], y1 = cedges[4 * e + 1], x2 = cedges[4 * e + 2], y2 = cedges[4 * e + 3];
                 const double ex = x2 - x1, ey = y2 - y1;
@@ -69,14 +73,18 @@ void or_overlap64(const double* pose, const float* poles, const uint32_t* off, u
                 else { qx = x1 + t * ex; qy = y1 + t * ey; }
                 const double dx = cx - qx, dy = cy - qy;
                 const double d = std::sqrt(dx * dx + dy * dy);
-                if (d < r) {
-                    Q += (r - d) * (r - d);
-                    if (grad && d > 0) {
-                        const double kk = -2.0 * (r - d) / d;
-                        grad[3 * a] += kk * dx;
-                        grad[3 * a + 1] += kk * dy;
-                        grad[3 * a + 2] += -kk * dx * ry + kk * dy * rx;
-                    }
+                if (d < best) { best = d; bx = dx; by = dy; }
+                if ((y1 > cy) != (y2 > cy) && cx < x1 + (cy - y1) * ex / ey) crossings++;
+            }
+            const double sgn = (crossings & 1u) ? 1.0 : -1.0;
+            const double pen = r - sgn * best;
+            if (pen > 0) {
+                Q += pen * pen;
+                if (grad && best > 0) {
+                    const double kk = -2.0 * pen * sgn / best;
+                    grad[3 * a] += kk * bx;
+                    grad[3 * a + 1] += kk * by;
+                    grad[3 * a + 2] += -kk * bx * ry + kk * by * rx;
                 }
             }
         }

@@ -72,3 +72,36 @@ def test_overlap_pairs_update(cuda):
     pose = torch.from_numpy(w["pose"]).to(cuda)
     q64, _ = O.overlap64(w["pose"].astype(np.float64), w["poles"], w["off"], w["pairs"], w["cedges"], grad=False)
     assert abs(ov.forward(pose).item() - q64) / q64 < 1e-5
+
+
+def test_overlap_pairs_from_the_quadtree(cuda):
+    """Pair candidates found on the GPU in the engine's quadtree (items as placed hazards):
+    deterministic, and Q over them equals Q over every pair of overlapping bounding circles
+    (the pairs they leave out contribute nothing); forward and backward against FP64.  Some
+    items are moved past the container walls (the signed container term)."""
+    w = S.config4_workload(seed=8, n_items=512, width=700.0, height=280.0, jitter=4.0)
+    w["pose"][[0, 2], 0] = [-8.0, 710.0]          # centres past the left and right walls
+    w["pose"][4, :2] = (350.0, -2.0)              # and past the bottom and top walls
+    w["pose"][5, :2] = (351.0, 285.0)
+    cde, keys = S.overlap_engine(w)
+    grid = J.candidate_pairs(w["pose"], w["rad"], [-20.0, -20.0, 720.0, 300.0], 8)
+    ov = J.PoleOverlap(w["poles"], w["off"], np.zeros((0, 2), np.uint32), w["cedges"])
+    pose = torch.from_numpy(w["pose"]).to(cuda)
+    n = ov.pairs_from_cde(cde, keys, pose, w["rad"])
+    pairs = ov.pairs()
+    assert n == len(pairs) > 0
+    assert (pairs[:, 0] < pairs[:, 1]).all()
+    gset = set(map(tuple, grid.tolist()))
+    assert set(map(tuple, pairs.tolist())) <= gset
+    assert ov.pairs_from_cde(cde, keys, pose, w["rad"]) == n and np.array_equal(ov.pairs(), pairs)
+    q64, g64 = O.overlap64(w["pose"].astype(np.float64), w["poles"], w["off"], pairs, w["cedges"])
+    q64g, g64g = O.overlap64(w["pose"].astype(np.float64), w["poles"], w["off"], grid, w["cedges"])
+    assert abs(q64 - q64g) <= 1e-12 * q64g and np.allclose(g64, g64g, rtol=0, atol=1e-9 * np.abs(g64g).max())
+    q = ov.forward(pose).item()
+    g = ov.backward(pose).cpu().numpy().astype(np.float64)
+    glob, scaled, comp = errors(g, g64)
+    print(f"quadtree pairs {n} (grid {len(grid)}); Q rel err {abs(q - q64) / q64:.2e}; grad L2 {glob:.2e}, "
+          f"max/scale {scaled:.2e}")
+    assert abs(q - q64) / q64 < 1e-5
+    assert glob < 1e-4 and scaled < 1e-4
+    assert np.array_equal(ov.backward(pose).cpu().numpy().astype(np.float64), g)

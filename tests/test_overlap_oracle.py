@@ -24,15 +24,33 @@ def test_two_single_pole_items_closed_form():
 
 
 def test_container_edge_closed_form():
+    """The container term is signed: D is the distance to the container boundary, positive
+    inside and negative outside, and a pole is penalised by (r - D)^2 when r > D."""
     poles = np.array([[1.0, 0.0, 2.0]], np.float32)
     off = np.array([0, 1], np.uint32)
-    pose = np.array([[0.5, 5.0, 0.0]])                           # pole centre at x = 1.5, wall at x = 0
-    ce = np.array([[0, 0, 0, 10]], np.float32)
-    q, g = O.overlap64(pose, poles, off, np.zeros((0, 2), np.uint32), ce)
+    ce = np.array([[0, 0, 10, 0], [10, 0, 10, 10], [10, 10, 0, 10], [0, 10, 0, 0]], np.float32)
+    none = np.zeros((0, 2), np.uint32)
+    # inside, 1.5 from the wall x = 0: (2 - 1.5)^2, pushed to +x
+    q, g = O.overlap64(np.array([[0.5, 5.0, 0.0]]), poles, off, none, ce)
     assert abs(q - 0.25) < 1e-12
     assert abs(g[0, 0] - (-1.0)) < 1e-12 and abs(g[0, 1]) < 1e-12
-    # torque: dc/dtheta = (-ry, rx) = (0, 1) at theta = 0 -> zero for a horizontal push
-    assert abs(g[0, 2]) < 1e-12
+    assert abs(g[0, 2]) < 1e-12       # dc/dtheta = (-ry, rx) = (0, 1) at theta = 0: no torque
+    # the centre 1 past the wall: D = -1, Q = (2 + 1)^2, dQ/dx = -2 (2 - x) = -6 at x = -1
+    q, g = O.overlap64(np.array([[-2.0, 5.0, 0.0]]), poles, off, none, ce)
+    assert abs(q - 9.0) < 1e-12
+    assert abs(g[0, 0] - (-6.0)) < 1e-12 and abs(g[0, 1]) < 1e-12
+    # wholly outside (5 past the wall): still penalised, (2 + 5)^2
+    q, _ = O.overlap64(np.array([[-6.0, 5.0, 0.0]]), poles, off, none, ce)
+    assert abs(q - 49.0) < 1e-12
+    # outside past a corner: D is minus the distance to the corner
+    q, g = O.overlap64(np.array([[-4.0, -3.0, 0.0]]), poles, off, none, ce)   # centre (-3, -3)
+    d = 3 * np.sqrt(2)
+    assert abs(q - (2 + d) ** 2) < 1e-9
+    k = -2 * (2 + d) * -1 / d
+    assert np.allclose(g[0, :2], [k * -3, k * -3])
+    # deep inside: no term
+    q, _ = O.overlap64(np.array([[4.0, 5.0, 0.0]]), poles, off, none, ce)
+    assert q == 0.0
 
 
 def test_gradient_matches_central_differences():
@@ -44,6 +62,7 @@ def test_gradient_matches_central_differences():
     pose = np.column_stack([rng.uniform(4, 26, n), rng.uniform(4, 16, n), rng.uniform(0, 6.28, n)])
     pairs = np.array([[i, j] for i in range(n) for j in range(i + 1, n)], np.uint32)
     ce = np.array([[0, 0, 30, 0], [30, 0, 30, 20], [30, 20, 0, 20], [0, 20, 0, 0]], np.float32)
+    pose[0, :2] = (-1.0, 8.0)                    # one item with its centre outside the container
     q, g = O.overlap64(pose, poles, off, pairs, ce)
     assert q > 0
     h = 1e-3
