@@ -172,6 +172,14 @@ def run_c2(args):
     log(f"batched: {ms:.3f} ms/step, {qps / 1e6:.1f} M queries/s, collision fraction {status_np.mean():.4f}")
     launch_ms = timed(lambda: plan.detect(mats_all, status), 5, stream)
     schedule = {k: v for k, v in tune["schedule"].items() if k != "flags"}
+    if args.profile == "collect":
+        c_, h_, k_ = plan.collect(mats_all, capacity=8)
+        ms_c = timed(lambda: plan.collect(mats_all, capacity=8, count=c_, hash=h_, keys=k_), args.steps, stream)
+        log(f"profile: {args.steps} batched collections, {ms_c:.3f} ms each")
+        return {"profile": "collect", "ms": ms_c, "steps": args.steps}
+    if args.profile == "overlap":
+        out = overlap_c4(stream)
+        return {"profile": "overlap", **out}
 
     result = base_result(args, {
         "workload": "single large layout: 4000x1500 rectangle + 5 holes, 2000 polygons (8-256 vertices), "
@@ -243,6 +251,12 @@ def run_c2(args):
             "collect_plain_queries_per_s": total / (ms_collect_plain / 1e3),
             "collect_plain_ms_per_step": ms_collect_plain, "collect_forms_agree": collect_forms_agree,
             "collect_over_collect_plain": ms_collect_plain / ms_collect}
+        cprof = pmc_profile("pmc_collect_config2.md")
+        if cprof:
+            ck = cprof["counters"]["k_collect_units"]
+            result["forms"]["collect_profile"] = {
+                "bound": cprof.get("bound"), "traffic": cprof.get("traffic"), "lane_util": ck["lane_util"],
+                "valu_issue": ck["valu_issue"], "wait_share": ck["wait_share"], "file": "profiles/pmc_collect_config2.md"}
         log(f"plain (per-set plans) {ms_plain_wave:.3f} ms/step, scalar {ms_scalar:.3f} ms, graph {ms_graph:.3f} ms; "
             f"same verdicts: {plain_wave_ok and scalar_ok and graph_ok}")
         result["overlap"] = overlap_c4(stream)
@@ -600,8 +614,14 @@ def run_c5(args, summary=False):
     poles = [p.poles for p in lay]
     off = np.zeros(len(poles) + 1, dtype=np.uint32)
     off[1:] = np.cumsum([len(p) for p in poles])
-    # overlap pose (x, y, theta) from each placed item's DTransformation (rotation, tx, ty)
-    pose = np.stack([(dt[1], dt[2], dt[0]) for _, dt in scene.items]).astype(np.float32)
+    # overlap pose (x, y, theta) from each placed item's DTransformation (rotation, tx, ty),
+    # jittered as one step of an optimiser would move the items (the placed layout itself has
+    # no overlap, so its Q and gradient are zero)
+    pose = np.stack([(dt[1], dt[2], dt[0]) for _, dt in scene.items]).astype(np.float64)
+    jit = np.random.default_rng(55)
+    pose[:, :2] += jit.normal(0.0, 1.0, (len(pose), 2))
+    pose[:, 2] += jit.normal(0.0, 0.02, len(pose))
+    pose = pose.astype(np.float32)
     rad = np.array([np.max(np.hypot(p[:, 0], p[:, 1]) + p[:, 2]) for p in poles], dtype=np.float32)
     cedges = np.array([[0, 0, 6000, 0], [6000, 0, 6000, 2000], [6000, 2000, 0, 2000], [0, 2000, 0, 0]], np.float32)
     ov = J.PoleOverlap(np.concatenate(poles), off, np.zeros((0, 2), np.uint32), cedges)
@@ -614,7 +634,8 @@ def run_c5(args, summary=False):
     ov.forward(pose_t, qb)
     ov.backward(pose_t, gb)
     ms_grad = timed(lambda: (ov.forward(pose_t, qb), ov.backward(pose_t, gb)), 20, stream)
-    log(f"c5 gradient pass: {ms_grad:.3f} ms ({n_pairs} candidate pairs from the quadtree in {t_pairs:.2f} s)")
+    log(f"c5 gradient pass: {ms_grad:.3f} ms ({n_pairs} candidate pairs from the quadtree in {t_pairs:.2f} s), "
+        f"Q = {qb.item():.4g}")
     # the three in one graph: captured on a side stream after one eager call each
     side = torch.cuda.Stream()
     status_g = torch.empty_like(status)
@@ -683,7 +704,8 @@ def run_c5(args, summary=False):
         g = gb.cpu().numpy().astype(np.float64)
         result["bit_exact_sample"] = {"queries": n, "verdicts_matching": n_det, "sets_matching": n_sets,
                                       "items": k_items}
-        result["overlap"]["q_rel_err_fp64"] = abs(qb.item() - q64) / q64
+        result["overlap"]["q_fp64"] = q64
+        result["overlap"]["q_rel_err_fp64"] = abs(qb.item() - q64) / max(q64, 1e-300)
         result["overlap"]["grad_rel_err_fp64"] = float(np.linalg.norm(g - g64) / np.linalg.norm(g64))
         result["cpu_baseline"] = {"value": n / sec, "unit": "queries/s", "cores": threads, "kind": "port",
                                   "sample": f"{n} queries (detect and collect) on the C++ oracle, {threads} threads",
@@ -715,6 +737,9 @@ def main():
     ap.add_argument("--c5-oracle", action="store_true",
                     help="c5: check 1M queries and the gradient against the oracle (minutes of CPU)")
     ap.add_argument("--quick", action="store_true", help="c2 only (no many-layout and full-scale summaries)")
+    ap.add_argument("--profile", choices=["", "collect", "overlap"], default="",
+                    help="c2: after the timed detection steps, run only `steps` batched collections "
+                         "(collect) or overlap evaluations (overlap), for counter runs of those kernels")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle (CPU baseline, bit-exact check)")
     ap.add_argument("--no-extra", action="store_true", help="only the batched form (for profiling)")
