@@ -324,7 +324,8 @@ def run_c2(args):
 def search_c2(cde, items, stream):
     """lbf's placement search for each of the query items on the layout, all at once on the
     GPU: 5000 samples per item (LBFConfig::default) as 8 uniform and 2 local rounds of 500,
-    continuous rotation, one captured graph per run."""
+    continuous rotation, one captured graph per run; then 32 independent searches per item
+    in one run, whose rounds fill the GPU like a detection step."""
     import jagua_amd as J
     search = J.PlacementSearch([(cde, it, "continuous", None) for it in items], samples_per_round=500,
                                uniform_rounds=8, local_rounds=2, seed=1)
@@ -333,8 +334,23 @@ def search_c2(cde, items, stream):
     res = search.results()
     n_samples = sum(r["samples"] for r in res)
     log(f"placement search: {len(items)} items x 5000 samples in {ms:.3f} ms, {sum(r['found'] for r in res)} placed")
-    return {"items": len(items), "samples_per_item": 5000, "ms_per_run": ms, "searches_per_s": len(items) / (ms / 1e3),
-            "samples_per_s": n_samples / (ms / 1e3), "placed": int(sum(r["found"] for r in res)), "form": "hipGraph"}
+    # the same searches 32 times over (2048 searches, each with its own sample stream): a round
+    # is then 1,024,000 queries, the size of one batched detection step
+    reps = 32
+    wide = J.PlacementSearch([(cde, it, "continuous", None) for _ in range(reps) for it in items],
+                             samples_per_round=500, uniform_rounds=8, local_rounds=2, seed=2)
+    wide.run(graph=True)
+    ms_w = timed(lambda: wide.run(graph=True), 5, stream)
+    res_w = wide.results()
+    n_w = sum(r["samples"] for r in res_w)
+    log(f"placement search, {len(res_w)} searches: {ms_w:.3f} ms, {n_w / ms_w / 1e3:.1f} M samples/s")
+    return {"searches": len(res_w), "samples_per_search": 5000, "ms_per_run": ms_w,
+            "searches_per_s": len(res_w) / (ms_w / 1e3), "samples_per_s": n_w / (ms_w / 1e3),
+            "placed": int(sum(r["found"] for r in res_w)), "form": "hipGraph",
+            "rounds": "8 uniform + 2 local, 500 samples per search each (1,024,000 queries per round)",
+            "searches_64": {"items": len(items), "samples_per_item": 5000, "ms_per_run": ms,
+                            "searches_per_s": len(items) / (ms / 1e3), "samples_per_s": n_samples / (ms / 1e3),
+                            "placed": int(sum(r["found"] for r in res))}}
 
 
 def update_c2(cde, scene, st0, n=100):
@@ -700,6 +716,7 @@ def run_c5(args, summary=False):
             n_det += int((r["status"] == st_np[a:b]).sum())
             n_sets += int(((r["count"] == c_np[a:b]) & (r["hash"] == h_np[a:b])).sum())
             sec += r["seconds"]
+            log(f"c5 oracle item {i + 1}/{k_items}: {n_det}/{n} verdicts, {n_sets}/{n} sets equal")
         q64, g64 = O.overlap64(pose.astype(np.float64), np.concatenate(poles), off, ov.pairs(), cedges)
         g = gb.cpu().numpy().astype(np.float64)
         result["bit_exact_sample"] = {"queries": n, "verdicts_matching": n_det, "sets_matching": n_sets,

@@ -372,14 +372,8 @@ __device__ __forceinline__ void collect_from(const CdeView& c, int start, float4
         if (nd.first_child >= 0 && !now) {
             fc = nd.first_child;
             const float4 nb = nd.bbox;
-            if (CIRCLE) {
-                uint32_t m = 0;
-                for (int k = 0; k < 4; k++) m |= circle_box(g.x, g.y, r, box_quadrant(nb, k)) ? (1u << k) : 0u;
-                return m;
-            }
-            float4 qs[4];
-            for (int k = 0; k < 4; k++) qs[k] = box_quadrant(nb, k);
-            return edge_quadrants(g, nb, qs);
+            if (CIRCLE) return circle_quadrants(g.x, g.y, r, nb);
+            return edge_quadrants_node(g, nb);
         }
         collect_now<CIRCLE>(c, nd, g, r, set);
         return 0u;

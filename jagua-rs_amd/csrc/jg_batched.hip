@@ -53,6 +53,11 @@ __global__ void k_reset_counters(uint32_t* counters) {
     for (uint32_t i = threadIdx.x; i < N_COUNTER_WORDS; i += blockDim.x) counters[i] = 0u;
 }
 
+// Hazard edges per edge-test item of the surrogate stage's traversals (pole circles and piers)
+// and of the polygon stage's (query edges).
+constexpr int SURR_EG = 4;
+constexpr int POLY_EG = 1;
+
 // The surrogate stage of one work unit (cd_engine.rs:204-223 for each of its queries): writes
 // the verdicts it settles and returns the mask of queries (bit i = query un.q0 + i) that go on
 // to the polygon stage.
@@ -104,7 +109,7 @@ __device__ __forceinline__ uint64_t surrogate_unit(const SetDesc& sd, const Unit
                 }
                 n = wave_push_roots(W, n, want, o, 0u, lane);
             }
-            wave_traverse<FILT, KIND, IPL>(c, f, W, n, lane);
+            wave_traverse<FILT, KIND, IPL, SURR_EG>(c, f, W, n, lane);
         }
         hit_mask = __ballot(W.hit[lane] != 0u) & valid;
     }
@@ -351,7 +356,7 @@ __device__ __forceinline__ void poly_queries(const SetDesc& sd, uint32_t q0, uin
                 }
                 n = wave_push_roots(W, n, want, o, (uint32_t)vr, lane);
             }
-            wave_traverse<FILT, KIND_EDGE>(c, f, W, n, lane);
+            wave_traverse<FILT, KIND_EDGE, 1, POLY_EG>(c, f, W, n, lane);
         }
         // containment over the virtual root's hazards (cd_engine.rs:176-193)
         wave_sync();
@@ -522,9 +527,12 @@ FilterView set_filter(const jg_query_set& s, const jg_hazard_filter& f, const ui
 jg_status table_image(const std::vector<jg_query_set>& sets, const std::vector<ItemView>& iv,
                       const std::vector<FilterView>& fv, const std::vector<std::pair<const uint32_t*, uint32_t>>& seeds,
                       const jg_plan_config& cfg, uint32_t n_cus, std::vector<uint8_t>& host, TableLayout& L) {
-    for (auto& s : sets)
+    for (auto& s : sets) {
         if (s.cde->view.n_nodes >= WL_MAX_NODES)
             return fail(JG_ERR_CAPACITY, "quadtree of more than 2^24 nodes in a batched plan");
+        if (s.cde->img.cap_edges > WL_MAX_EDGES)
+            return fail(JG_ERR_CAPACITY, "engine image of more than 2^29 hazard edges in a batched plan");
+    }
     std::vector<SetDesc> sd(sets.size());
     bool any_filter = false, any_pole = false, any_pier = false;
     L.max_slot_words = 1;

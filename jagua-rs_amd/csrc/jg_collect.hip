@@ -240,11 +240,7 @@ __device__ __forceinline__ void wave_collect(const CdeView& c, CollectLds& L, ui
             if (live) {
                 if (nd.first_child >= 0 && nd.n_active_edges > c.cd_threshold) {
                     fc = nd.first_child;
-                    if (circle) {
-                        for (int k = 0; k < 4; k++) m |= circle_box(g.x, g.y, g.z, box_quadrant(nd.bbox, k)) ? (1u << k) : 0u;
-                    } else {
-                        m = edge_quadrants(g, nd.bbox);
-                    }
+                    m = circle ? circle_quadrants(g.x, g.y, g.z, nd.bbox) : edge_quadrants_node(g, nd.bbox);
                 } else if (INLINE_EDGES) {
                     // the visit inserts its Entire hazards and tests its Partial hazards' edges itself
                     for (uint32_t i = nd.haz_begin; i < nd.haz_begin + cnt; i++) {

@@ -126,11 +126,9 @@ __device__ __forceinline__ int virtual_root(const CdeView& c, float4 bb) {
     for (;;) {
         const DevNode nd = c.nodes[n];
         if (nd.first_child < 0) return n;
-        int next = -1;
-        for (int k = 0; k < 4; k++)
-            if (box_relation(box_quadrant(nd.bbox, k), bb) == REL_SURROUNDING) { next = nd.first_child + k; break; }
-        if (next < 0) return n;
-        n = next;
+        const int k = quadrant_surrounding(nd.bbox, bb);
+        if (k < 0) return n;
+        n = nd.first_child + k;
     }
 }
 
@@ -145,9 +143,7 @@ __device__ __forceinline__ bool collides_circle(const CdeView& c, int start, flo
         if (entire) return HIT;
         if (nd.first_child >= 0) {
             fc = nd.first_child;
-            uint32_t m = 0;
-            for (int k = 0; k < 4; k++) m |= circle_box(cx, cy, r, box_quadrant(nd.bbox, k)) ? (1u << k) : 0u;
-            return m;
+            return circle_quadrants(cx, cy, r, nd.bbox);
         }
         const int he = (int)(nd.haz_begin + (nd.haz_info & COUNT_MASK));
         for (int i = s; i < he; i++) {
@@ -171,9 +167,7 @@ __device__ __forceinline__ bool collides_edge(const CdeView& c, int start, float
         if (entire) return HIT;
         if (nd.first_child >= 0) {
             fc = nd.first_child;
-            float4 qs[4];
-            for (int k = 0; k < 4; k++) qs[k] = box_quadrant(nd.bbox, k);
-            return edge_quadrants(ed, nd.bbox, qs);
+            return edge_quadrants_node(ed, nd.bbox);
         }
         const int he = (int)(nd.haz_begin + (nd.haz_info & COUNT_MASK));
         for (int i = s; i < he; i++) {
@@ -219,10 +213,14 @@ static_assert(WL_NCAP >= WL_ENT + WL_RESERVE + 3u, "node queue must hold the roo
 constexpr uint32_t WL_NODE_MASK = 0xffffffu;
 constexpr uint32_t WL_MAX_NODES = 1u << 24;
 constexpr uint32_t WL_CIRCLE = 0x80u;     // WorkList::ent_slot flag: the entity is a circle
+// An edge-test entry names up to 8 consecutive hazard edges: the first edge's index in its low
+// 29 bits, the count less one in the top 3.
+constexpr uint32_t WL_EDGE_INDEX = 0x1fffffffu;
+constexpr uint32_t WL_MAX_EDGES = WL_EDGE_INDEX + 1u;
 
 struct WorkList {                         // per-wave LDS
     uint32_t* nodes;                      // [WL_NCAP] entity << 24 | node
-    uint32_t* edges;                      // [WL_ECAP] hazard edge index
+    uint32_t* edges;                      // [WL_ECAP] hazard edges (WL_EDGE_INDEX)
     uint8_t* edge_ent;                    // [WL_ECAP] entity of the edge test
     float4* geom;                         // [WL_ENT] circle (cx, cy, r, -) or edge (x1 y1 x2 y2)
     uint8_t* ent_slot;                    // [WL_ENT] query slot (0..63) | WL_CIRCLE
@@ -280,14 +278,8 @@ __device__ __forceinline__ uint32_t visit_node(const CdeView& c, int n, bool cir
     if (entire) return HIT;
     if (nd.first_child >= 0) {
         fc = nd.first_child;
-        if (circle) {
-            uint32_t m = 0;
-            for (int k = 0; k < 4; k++) m |= circle_box(g.x, g.y, g.z, box_quadrant(nd.bbox, k)) ? (1u << k) : 0u;
-            return m;
-        }
-        float4 qs[4];
-        for (int k = 0; k < 4; k++) qs[k] = box_quadrant(nd.bbox, k);
-        return edge_quadrants(g, nd.bbox, qs);
+        if (circle) return circle_quadrants(g.x, g.y, g.z, nd.bbox);
+        return edge_quadrants_node(g, nd.bbox);
     }
     return leaf_hits<FILT>(c, nd, s, circle, g, f, q) ? HIT : 0u;
 }
@@ -348,11 +340,15 @@ __device__ __forceinline__ uint32_t wave_push_roots(const WorkList& W, uint32_t 
 // W.hit[slot] is set for every query slot one of whose entities collides.  Callers push at
 // most WL_ENT roots (WL_ENT <= WL_NCAP).  Each lane handles IPL items per step (node visits or
 // edge tests): their memory reads are issued together, so a step waits for one round trip
-// instead of IPL.
-template <bool FILT, int KIND, int IPL = 1>
+// instead of IPL.  An edge-test item holds up to EG consecutive edges of one hazard entry: a
+// leaf visit pushes one item per EG edges, so the lanes of a node step loop over a few pushes
+// each instead of one per edge (a leaf of the benchmark layout holds about 7 edges a circle
+// may meet), and an edge step tests up to EG edges per item.
+template <bool FILT, int KIND, int IPL = 1, int EG = 1>
 __device__ __forceinline__ void wave_traverse(const CdeView& c, const FilterView& f, const WorkList& W, uint32_t nn,
                                               uint32_t lane) {
     static_assert(IPL == 1 || IPL == 2, "one or two items per lane");
+    static_assert(EG >= 1 && EG <= 8, "one to eight edges per edge-test item");
     // edge tests one item may push; the per-lane sum over IPL items stays below 2^16 / 64
     constexpr uint32_t EDGE_LIMIT = IPL == 1 ? (1u << WL_EDGE_BITS) : (1u << (WL_EDGE_BITS - 1));
     static_assert(64u * IPL * (EDGE_LIMIT - 1u) < 65536u, "a wave's edge pushes must fit 16 bits");
@@ -363,14 +359,18 @@ __device__ __forceinline__ void wave_traverse(const CdeView& c, const FilterView
         if (ne >= STEP || nn == 0u) {
             // edge step: both tests' edges are read before either is evaluated
             const uint32_t take = ne < STEP ? ne : STEP;
-            uint32_t eo[IPL];
-            float4 hedge[IPL];
+            uint32_t eo[IPL], cnt[IPL];
+            float4 hedge[IPL][EG];
 #pragma unroll
             for (int r = 0; r < IPL; r++) {
                 const uint32_t idx = lane + 64u * (uint32_t)r;
                 const uint32_t at = idx < take ? ne - 1u - idx : 0u;
                 eo[r] = W.edge_ent[at];
-                hedge[r] = c.edges[idx < take ? W.edges[at] : 0u];
+                const uint32_t v = idx < take ? W.edges[at] : 0u;
+                const uint32_t e0 = v & WL_EDGE_INDEX;
+                cnt[r] = (v >> 29) + 1u;
+#pragma unroll
+                for (int k = 0; k < EG; k++) hedge[r][k] = c.edges[e0 + ((uint32_t)k < cnt[r] ? (uint32_t)k : 0u)];
             }
 #pragma unroll
             for (int r = 0; r < IPL; r++) {
@@ -379,7 +379,11 @@ __device__ __forceinline__ void wave_traverse(const CdeView& c, const FilterView
                 const uint32_t slot = es & 63u;
                 if (W.hit[slot]) continue;
                 const bool circle = KIND == KIND_CIRCLE || (KIND == KIND_MIXED && (es & WL_CIRCLE) != 0u);
-                if (edge_hit(circle, W.geom[eo[r]], hedge[r])) W.hit[slot] = 1u;
+                const float4 gm = W.geom[eo[r]];
+                bool h = false;
+#pragma unroll
+                for (int k = 0; k < EG; k++) h = h || ((uint32_t)k < cnt[r] && edge_hit(circle, gm, hedge[r][k]));
+                if (h) W.hit[slot] = 1u;
             }
             ne -= take;
             continue;
@@ -435,12 +439,8 @@ __device__ __forceinline__ void wave_traverse(const CdeView& c, const FilterView
                         s[r] = -1;
                     } else if (s[r] >= 0 && nd[r].first_child >= 0) {
                         fc[r] = nd[r].first_child;
-                        if (circle[r]) {
-                            for (int k = 0; k < 4; k++)
-                                m[r] |= circle_box(g[r].x, g[r].y, g[r].z, box_quadrant(nd[r].bbox, k)) ? (1u << k) : 0u;
-                        } else {
-                            m[r] = edge_quadrants(g[r], nd[r].bbox);
-                        }
+                        m[r] = circle[r] ? circle_quadrants(g[r].x, g[r].y, g[r].z, nd[r].bbox)
+                                         : edge_quadrants_node(g[r], nd[r].bbox);
                         s[r] = -1;
                     } else if (s[r] >= 0) {
                         // leaf: count the edge tests of its relevant Partial hazards
@@ -448,7 +448,7 @@ __device__ __forceinline__ void wave_traverse(const CdeView& c, const FilterView
                         for (int i = s[r]; i < he[r]; i++) {
                             const DevEntry e = c.ent[i];
                             if (!irrel<FILT>(f, q[r], e.slot_flags & SLOT_MASK) && ff_pass(e, circle[r], g[r]))
-                                n_edges[r] += e.edge_count;
+                                n_edges[r] += (e.edge_count + (uint32_t)EG - 1u) / (uint32_t)EG;
                         }
                     }
                 }
@@ -491,8 +491,9 @@ __device__ __forceinline__ void wave_traverse(const CdeView& c, const FilterView
                 for (int i = s[r]; i < he[r]; i++) {
                     const DevEntry e = c.ent[i];
                     if (irrel<FILT>(f, q[r], e.slot_flags & SLOT_MASK) || !ff_pass(e, circle[r], g[r])) continue;
-                    for (uint32_t j = 0; j < e.edge_count; j++, at++) {
-                        W.edges[at] = e.edge_begin + j;
+                    for (uint32_t j = 0; j < e.edge_count; j += EG, at++) {
+                        const uint32_t k = e.edge_count - j < (uint32_t)EG ? e.edge_count - j : (uint32_t)EG;
+                        W.edges[at] = (e.edge_begin + j) | ((k - 1u) << 29);
                         W.edge_ent[at] = (uint8_t)o;
                     }
                 }

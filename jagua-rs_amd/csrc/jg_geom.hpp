@@ -157,6 +157,63 @@ JG_HD uint32_t edge_quadrants(float4 e, float4 r) {
     return edge_quadrants_sides(e, r, st);
 }
 
+// Quadrant intervals of a box r with r.x <= r.z and r.y <= r.w (every quadtree node's box,
+// each built by box_quadrant from its parent's).  The rounded midpoint m.x = (r.x + r.z) / 2
+// lies in [r.x, r.z], so the min / max of box_quadrant(r, k) are plain bounds: quadrant k spans
+// x in [m.x, r.z] for k = 0, 3 and [r.x, m.x] for k = 1, 2, and y in [m.y, r.w] for k = 0, 1
+// and [r.y, m.y] for k = 2, 3.  The quadrant tests below evaluate the same comparisons and
+// arithmetic on these bounds as the per-quadrant forms, sharing the two x and two y
+// intervals instead of building four boxes: their results are identical for every entity.
+
+// edge_quadrants for a node box (qt_traits.rs:19-106).
+JG_HD uint32_t edge_quadrants_node(float4 e, float4 r) {
+    const float ex0 = mn(e.x, e.z), ex1 = mx(e.x, e.z), ey0 = mn(e.y, e.w), ey1 = mx(e.y, e.w);
+    const float mxm = (r.x + r.z) / 2.0f, mym = (r.y + r.w) / 2.0f;
+    // box overlap per interval (rect.rs:301-345 first test) and endpoint containment (rect.rs:282-288)
+    const bool oxr = !(mx(ex0, mxm) > mn(ex1, r.z)), oxl = !(mx(ex0, r.x) > mn(ex1, mxm));
+    const bool oyt = !(mx(ey0, mym) > mn(ey1, r.w)), oyb = !(mx(ey0, r.y) > mn(ey1, mym));
+    const bool axr = e.x >= mxm && e.x <= r.z, axl = e.x >= r.x && e.x <= mxm;
+    const bool ayt = e.y >= mym && e.y <= r.w, ayb = e.y >= r.y && e.y <= mym;
+    const bool bxr = e.z >= mxm && e.z <= r.z, bxl = e.z >= r.x && e.z <= mxm;
+    const bool byt = e.w >= mym && e.w <= r.w, byb = e.w >= r.y && e.w <= mym;
+    const bool ov[4] = {oxr && oyt, oxl && oyt, oxl && oyb, oxr && oyb};
+    const bool pt[4] = {(axr && ayt) || (bxr && byt), (axl && ayt) || (bxl && byt), (axl && ayb) || (bxl && byb),
+                        (axr && ayb) || (bxr && byb)};
+    uint32_t st = 0;
+    for (int k = 0; k < 4; k++) {
+        if (!ov[k]) st |= 1u << k;
+        else if (pt[k]) st |= (1u << k) | (1u << (k + 4));
+    }
+    if ((st & 0xfu) == 0xfu) return st >> 4;
+    return edge_quadrants_sides(e, r, st);
+}
+
+// circle_box against the four quadrants of a node box (circle.rs:115-127): bit k = quadrant k.
+JG_HD uint32_t circle_quadrants(float cx, float cy, float rad, float4 r) {
+    const float mxm = (r.x + r.z) / 2.0f, mym = (r.y + r.w) / 2.0f;
+    const float dxr = mx(mxm, mn(cx, r.z)) - cx, dxl = mx(r.x, mn(cx, mxm)) - cx;
+    const float dyt = mx(mym, mn(cy, r.w)) - cy, dyb = mx(r.y, mn(cy, mym)) - cy;
+    const float xr = dxr * dxr, xl = dxl * dxl, yt = dyt * dyt, yb = dyb * dyb;
+    const float rr = rad * rad;
+    return (xr + yt <= rr ? 1u : 0u) | (xl + yt <= rr ? 2u : 0u) | (xl + yb <= rr ? 4u : 0u) | (xr + yb <= rr ? 8u : 0u);
+}
+
+// get_virtual_root's step (cd_engine.rs:314-327): the first quadrant k of a node box r whose
+// box_relation to bb is Surrounding (rect.rs:46-65), or -1.  For a box bb with bb.x <= bb.z
+// and bb.y <= bb.w, surrounding implies overlap, so the test is four bound comparisons per
+// quadrant; any other bb takes the per-quadrant relation.
+JG_HD int quadrant_surrounding(float4 r, float4 bb) {
+    if (!(bb.x <= bb.z && bb.y <= bb.w)) {
+        for (int k = 0; k < 4; k++)
+            if (box_relation(box_quadrant(r, k), bb) == REL_SURROUNDING) return k;
+        return -1;
+    }
+    const float mxm = (r.x + r.z) / 2.0f, mym = (r.y + r.w) / 2.0f;
+    const bool xr = mxm <= bb.x && r.z >= bb.z, xl = r.x <= bb.x && mxm >= bb.z;
+    const bool yt = mym <= bb.y && r.w >= bb.w, yb = r.y <= bb.y && mym >= bb.w;
+    return (xr && yt) ? 0 : (xl && yt) ? 1 : (xl && yb) ? 2 : (xr && yb) ? 3 : -1;
+}
+
 JG_HD uint32_t edge_quadrants(float4 e, float4 r, const float4* q) {
     float ex0 = mn(e.x, e.z), ex1 = mx(e.x, e.z), ey0 = mn(e.y, e.w), ey1 = mx(e.y, e.w);
     uint32_t st = 0;

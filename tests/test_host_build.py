@@ -206,3 +206,23 @@ def test_image_patches_equal_fresh_flatten(tmp_path, seed, depth):
     assert out.returncode == 0, out.stderr
     ok, patches, builds = out.stdout.split()
     assert ok == "ok" and int(patches) > 100
+
+
+def test_shared_interval_quadrant_forms(tmp_path):
+    """The device code's quadrant tests (edge_quadrants_node, circle_quadrants,
+    quadrant_surrounding in jg_geom.hpp) return exactly what the per-quadrant forms of the
+    reference's tests return (edge_quadrants over box_quadrant boxes, circle_box and
+    box_relation per quadrant), on node boxes subdivided as the quadtree does and entities on
+    and one ulp around the bounds and midlines (tests/quadrant_forms_check.cpp)."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = tmp_path / "quadrant_forms_check"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__",
+                    "-I/opt/rocm/include", os.path.join(root, "tests", "quadrant_forms_check.cpp"), "-o", str(exe)],
+                   check=True, capture_output=True)
+    for seed in (1, 2):
+        out = subprocess.run([str(exe), str(seed), "500000"], capture_output=True, text=True)
+        assert out.returncode == 0, out.stdout + out.stderr
+        ok, cases, on_mid = out.stdout.split()
+        assert ok == "ok" and int(on_mid) > 100000
