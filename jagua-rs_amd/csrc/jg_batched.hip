@@ -92,7 +92,7 @@ __device__ __forceinline__ uint64_t surrogate_unit(const SetDesc& sd, const Unit
                     const uint32_t qi = e / n_ent, k = e - qi * n_ent;
                     if (!W.hit[qi]) {
                         const Mat m = load_mat(m6, base + qi);
-                        const float4 p = it.ff[k];
+                        const float4 p = ldg4(it.ff + k);
                         const bool circle = k < it.n_poles;
                         float4 gm;
                         if (circle) {
@@ -177,12 +177,12 @@ __device__ __forceinline__ bool scan_containment(const CdeView& c, const FilterV
 #pragma unroll
         for (uint32_t r = 0; r < CU; r++) {
             const uint32_t i = k0 + r * G + sub;
-            sl[r] = c.ent[e0 + (i < cnt ? i : 0u)].slot_flags & SLOT_MASK;
+            sl[r] = ldg_slot_flags(c.ent + e0 + (i < cnt ? i : 0u)) & SLOT_MASK;
         }
 #pragma unroll
         for (uint32_t r = 0; r < CU; r++) {
-            hb[r] = c.haz_bbox[sl[r]];
-            iw[r] = c.haz_poi[sl[r]].w;
+            hb[r] = ldg4(c.haz_bbox + sl[r]);
+            iw[r] = ldg4(c.haz_poi + sl[r]).w;
         }
         bool direct = false;               // exterior scope, no bbox relation: !contained
         uint32_t pend = 0;                 // bit r: candidate r needs its point-in-polygon test
@@ -206,10 +206,10 @@ __device__ __forceinline__ bool scan_containment(const CdeView& c, const FilterV
                 if (rl == REL_SURROUNDING) {
                     contained = pip_hazard_group(c, s2, qpoi.x, qpoi.y, sub, G);
                 } else {
-                    const float4 hp = c.haz_poi[s2];
+                    const float4 hp = ldg4(c.haz_poi + s2);
                     contained = pip_query_group(it, m, bb, hp.x, hp.y, sub, G);
                 }
-                if (c.haz_poi[s2].w != 0.0f ? contained : !contained) return true;
+                if (ldg4(c.haz_poi + s2).w != 0.0f ? contained : !contained) return true;
             }
         }
     }
@@ -237,12 +237,12 @@ __device__ __forceinline__ bool scan_containment_wave(const CdeView& c, const Fi
 #pragma unroll
         for (uint32_t r = 0; r < CU; r++) {
             const uint32_t i = k0 + r * 64u + lane;
-            sl[r] = c.ent[e0 + (i < cnt ? i : 0u)].slot_flags & SLOT_MASK;
+            sl[r] = ldg_slot_flags(c.ent + e0 + (i < cnt ? i : 0u)) & SLOT_MASK;
         }
 #pragma unroll
         for (uint32_t r = 0; r < CU; r++) {
-            hb[r] = c.haz_bbox[sl[r]];
-            iw[r] = c.haz_poi[sl[r]].w;
+            hb[r] = ldg4(c.haz_bbox + sl[r]);
+            iw[r] = ldg4(c.haz_poi + sl[r]).w;
         }
         bool direct = false;               // exterior scope, no bbox relation: !contained
         uint32_t pend = 0;                 // bit r: candidate r needs its point-in-polygon test
@@ -266,10 +266,10 @@ __device__ __forceinline__ bool scan_containment_wave(const CdeView& c, const Fi
                 if (rl == REL_SURROUNDING) {
                     contained = pip_hazard_wave(c, s, qpoi.x, qpoi.y, lane);
                 } else {
-                    const float4 hp = c.haz_poi[s];
+                    const float4 hp = ldg4(c.haz_poi + s);
                     contained = pip_query_wave(it, m, bb, hp.x, hp.y, lane);
                 }
-                if (c.haz_poi[s].w != 0.0f ? contained : !contained) return true;
+                if (ldg4(c.haz_poi + s).w != 0.0f ? contained : !contained) return true;
             }
         }
     }
@@ -289,7 +289,7 @@ __device__ __forceinline__ void poly_queries(const SetDesc& sd, uint32_t q0, uin
     const uint32_t per = 64u / G;                     // queries per pass
     const uint32_t slot = lane / G, sub = lane % G;
     const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
-    const float4 root_bb = c.nodes[0].bbox;
+    const float4 root_bb = ldg_node(c.nodes).bbox;
     const uint32_t nv = it.n_v;
     uint64_t todo = mask;
     while (todo) {
@@ -306,7 +306,8 @@ __device__ __forceinline__ void poly_queries(const SetDesc& sd, uint32_t q0, uin
         float4 bb = make_float4(3.40282347e+38f, 3.40282347e+38f, -3.40282347e+38f, -3.40282347e+38f);
         if (active)
             for (uint32_t i = sub; i < nv; i += G) {
-                const float2 p = xform(m, it.v[i].x, it.v[i].y);
+                const float2 vi = ldg2(it.v + i);
+                const float2 p = xform(m, vi.x, vi.y);
                 bb.x = mn(bb.x, p.x); bb.y = mn(bb.y, p.y); bb.z = mx(bb.z, p.x); bb.w = mx(bb.w, p.y);
             }
         for (uint32_t o = G >> 1; o > 0; o >>= 1) {
@@ -349,8 +350,9 @@ __device__ __forceinline__ void poly_queries(const SetDesc& sd, uint32_t q0, uin
                 const bool want = active && i < nv && W.hit[slot] == 0u;
                 if (want) {
                     const uint32_t k = i + 1 == nv ? 0 : i + 1;
-                    const float2 a = xform(m, it.v[i].x, it.v[i].y);
-                    const float2 b2 = xform(m, it.v[k].x, it.v[k].y);
+                    const float2 vi = ldg2(it.v + i), vk = ldg2(it.v + k);
+                    const float2 a = xform(m, vi.x, vi.y);
+                    const float2 b2 = xform(m, vk.x, vk.y);
                     W.geom[o] = make_float4(a.x, a.y, b2.x, b2.y);
                     W.ent_slot[o] = (uint8_t)slot;
                 }
@@ -365,7 +367,7 @@ __device__ __forceinline__ void poly_queries(const SetDesc& sd, uint32_t q0, uin
         bool hit = false, heavy = false;
         uint32_t e0 = 0, cntv = 0;
         if (!done) {
-            const DevNode vn = c.nodes[vr];
+            const DevNode vn = ldg_node(c.nodes + vr);
             cntv = vn.haz_info & COUNT_MASK;
             e0 = vn.haz_begin;
             if (cntv > 0 && (vn.haz_info & ENTIRE_BIT)) {

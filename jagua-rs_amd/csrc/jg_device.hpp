@@ -24,6 +24,61 @@ struct DevEntry {                     // mirrors FlatEntry (32 B)
     uint32_t slot_flags, edge_begin, edge_count, n_edges_total;
 };
 
+// Reads of the engine image and the item records.  Their pointers come out of descriptor
+// tables, so the compiler sees generic addresses and would issue flat loads, which count
+// against the LDS wait counter as well: every LDS read of the traversal would then wait for
+// the node and edge loads in flight.  These read through global-address-space pointers
+// (global_load) instead.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef int jg_i4 __attribute__((ext_vector_type(4)));
+typedef int jg_i2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) const jg_i4 jg_gi4;
+typedef __attribute__((address_space(1))) const jg_i2 jg_gi2;
+typedef __attribute__((address_space(1))) const int jg_gi1;
+__device__ __forceinline__ float4 ldg4(const float4* p) {
+    const jg_i4 v = *(jg_gi4*)p;
+    return make_float4(__int_as_float(v.x), __int_as_float(v.y), __int_as_float(v.z), __int_as_float(v.w));
+}
+__device__ __forceinline__ float2 ldg2(const float2* p) {
+    const jg_i2 v = *(jg_gi2*)p;
+    return make_float2(__int_as_float(v.x), __int_as_float(v.y));
+}
+__device__ __forceinline__ uint2 ldg2u(const uint2* p) {
+    const jg_i2 v = *(jg_gi2*)p;
+    return make_uint2((uint32_t)v.x, (uint32_t)v.y);
+}
+__device__ __forceinline__ uint32_t ldg1(const uint32_t* p) { return (uint32_t)*(jg_gi1*)p; }
+__device__ __forceinline__ DevNode ldg_node(const DevNode* p) {
+    const jg_i4 a = ((jg_gi4*)p)[0], b = ((jg_gi4*)p)[1];
+    DevNode n;
+    n.bbox = make_float4(__int_as_float(a.x), __int_as_float(a.y), __int_as_float(a.z), __int_as_float(a.w));
+    n.first_child = b.x;
+    n.haz_begin = (uint32_t)b.y;
+    n.haz_info = (uint32_t)b.z;
+    n.n_active_edges = (uint32_t)b.w;
+    return n;
+}
+__device__ __forceinline__ DevEntry ldg_ent(const DevEntry* p) {
+    const jg_i4 a = ((jg_gi4*)p)[0], b = ((jg_gi4*)p)[1];
+    DevEntry e;
+    e.ff = make_float4(__int_as_float(a.x), __int_as_float(a.y), __int_as_float(a.z), __int_as_float(a.w));
+    e.slot_flags = (uint32_t)b.x;
+    e.edge_begin = (uint32_t)b.y;
+    e.edge_count = (uint32_t)b.z;
+    e.n_edges_total = (uint32_t)b.w;
+    return e;
+}
+#else
+__device__ inline float4 ldg4(const float4* p) { return *p; }
+__device__ inline float2 ldg2(const float2* p) { return *p; }
+__device__ inline uint2 ldg2u(const uint2* p) { return *p; }
+__device__ inline uint32_t ldg1(const uint32_t* p) { return *p; }
+__device__ inline DevNode ldg_node(const DevNode* p) { return *p; }
+__device__ inline DevEntry ldg_ent(const DevEntry* p) { return *p; }
+#endif
+// The slot | Entire word of an entry (its fifth word).
+__device__ __forceinline__ uint32_t ldg_slot_flags(const DevEntry* p) { return ldg1((const uint32_t*)p + 4); }
+
 struct CdeView {
     const DevNode* nodes;
     const uint32_t* node_up;          // parent << 2 | sibling index
@@ -124,7 +179,7 @@ __device__ __forceinline__ bool descend(const CdeView& c, int start, Visit visit
 __device__ __forceinline__ int virtual_root(const CdeView& c, float4 bb) {
     int n = 0;
     for (;;) {
-        const DevNode nd = c.nodes[n];
+        const DevNode nd = ldg_node(c.nodes + n);
         if (nd.first_child < 0) return n;
         const int k = quadrant_surrounding(nd.bbox, bb);
         if (k < 0) return n;
@@ -370,7 +425,7 @@ __device__ __forceinline__ void wave_traverse(const CdeView& c, const FilterView
                 const uint32_t e0 = v & WL_EDGE_INDEX;
                 cnt[r] = (v >> 29) + 1u;
 #pragma unroll
-                for (int k = 0; k < EG; k++) hedge[r][k] = c.edges[e0 + ((uint32_t)k < cnt[r] ? (uint32_t)k : 0u)];
+                for (int k = 0; k < EG; k++) hedge[r][k] = ldg4(c.edges + e0 + ((uint32_t)k < cnt[r] ? (uint32_t)k : 0u));
             }
 #pragma unroll
             for (int r = 0; r < IPL; r++) {
@@ -410,7 +465,7 @@ __device__ __forceinline__ void wave_traverse(const CdeView& c, const FilterView
         for (int r = 0; r < IPL; r++) {                // the node records, all in flight together
             const uint32_t idx = lane + 64u * (uint32_t)r;
             item[r] = idx < take ? W.nodes[nn - 1u - idx] : 0u;
-            nd[r] = c.nodes[item[r] & WL_NODE_MASK];
+            nd[r] = ldg_node(c.nodes + (item[r] & WL_NODE_MASK));
         }
         uint32_t packed = 0;
 #pragma unroll
@@ -446,7 +501,7 @@ __device__ __forceinline__ void wave_traverse(const CdeView& c, const FilterView
                         // leaf: count the edge tests of its relevant Partial hazards
                         he[r] = (int)(nd[r].haz_begin + (nd[r].haz_info & COUNT_MASK));
                         for (int i = s[r]; i < he[r]; i++) {
-                            const DevEntry e = c.ent[i];
+                            const DevEntry e = ldg_ent(c.ent + i);
                             if (!irrel<FILT>(f, q[r], e.slot_flags & SLOT_MASK) && ff_pass(e, circle[r], g[r]))
                                 n_edges[r] += (e.edge_count + (uint32_t)EG - 1u) / (uint32_t)EG;
                         }
@@ -457,9 +512,9 @@ __device__ __forceinline__ void wave_traverse(const CdeView& c, const FilterView
                 // more edge tests than one push carries: this lane tests the leaf's edges itself
                 bool h = false;
                 for (int i = s[r]; i < he[r] && !h; i++) {
-                    const DevEntry e = c.ent[i];
+                    const DevEntry e = ldg_ent(c.ent + i);
                     if (irrel<FILT>(f, q[r], e.slot_flags & SLOT_MASK) || !ff_pass(e, circle[r], g[r])) continue;
-                    for (uint32_t j = 0; j < e.edge_count && !h; j++) h = edge_hit(circle[r], g[r], c.edges[e.edge_begin + j]);
+                    for (uint32_t j = 0; j < e.edge_count && !h; j++) h = edge_hit(circle[r], g[r], ldg4(c.edges + e.edge_begin + j));
                 }
                 if (h) W.hit[slot[r]] = 1u;
                 n_edges[r] = 0;
@@ -489,7 +544,7 @@ __device__ __forceinline__ void wave_traverse(const CdeView& c, const FilterView
             if (fits) {
                 const uint32_t o = item[r] >> 24;
                 for (int i = s[r]; i < he[r]; i++) {
-                    const DevEntry e = c.ent[i];
+                    const DevEntry e = ldg_ent(c.ent + i);
                     if (irrel<FILT>(f, q[r], e.slot_flags & SLOT_MASK) || !ff_pass(e, circle[r], g[r])) continue;
                     for (uint32_t j = 0; j < e.edge_count; j += EG, at++) {
                         const uint32_t k = e.edge_count - j < (uint32_t)EG ? e.edge_count - j : (uint32_t)EG;
@@ -501,9 +556,9 @@ __device__ __forceinline__ void wave_traverse(const CdeView& c, const FilterView
                 // edge queue full: this lane tests the leaf's edges itself
                 bool h = false;
                 for (int i = s[r]; i < he[r] && !h; i++) {
-                    const DevEntry e = c.ent[i];
+                    const DevEntry e = ldg_ent(c.ent + i);
                     if (irrel<FILT>(f, q[r], e.slot_flags & SLOT_MASK) || !ff_pass(e, circle[r], g[r])) continue;
-                    for (uint32_t j = 0; j < e.edge_count && !h; j++) h = edge_hit(circle[r], g[r], c.edges[e.edge_begin + j]);
+                    for (uint32_t j = 0; j < e.edge_count && !h; j++) h = edge_hit(circle[r], g[r], ldg4(c.edges + e.edge_begin + j));
                 }
                 if (h) W.hit[slot[r]] = 1u;
             }
@@ -570,14 +625,14 @@ __device__ __forceinline__ int wave_sum_i32(int v) {
 // edges l, l + 64, ...).  Each edge's crossing count is the sequential loop's, and their sum is
 // an integer, so the parity is the same.  All 64 lanes call them with the same arguments.
 __device__ __forceinline__ bool pip_hazard_wave(const CdeView& c, uint32_t slot, float px, float py, uint32_t lane) {
-    const float4 hb = c.haz_bbox[slot];
+    const float4 hb = ldg4(c.haz_bbox + slot);
     if (!box_has_point(hb, px, py)) return false;
     const float ray_x = hb.z + (hb.z - hb.x);
-    const uint2 vr = c.haz_vr[slot];
+    const uint2 vr = ldg2u(c.haz_vr + slot);
     const float2* v = c.haz_verts + vr.x;
     int cnt = 0;
     for (uint32_t i = lane; i < vr.y; i += 64u) {
-        const float2 a = v[i], b = v[i + 1 == vr.y ? 0 : i + 1];
+        const float2 a = ldg2(v + i), b = ldg2(v + (i + 1 == vr.y ? 0 : i + 1));
         cnt += pip_edge(px, py, ray_x, a.x, a.y, b.x, b.y);
     }
     return (wave_sum_i32(cnt) & 1) == 1;
@@ -589,7 +644,8 @@ __device__ __forceinline__ bool pip_query_wave(const ItemView& it, const Mat& m,
     int cnt = 0;
     for (uint32_t i = lane; i < it.n_v; i += 64u) {
         const uint32_t k = i + 1 == it.n_v ? 0 : i + 1;
-        const float2 a = xform(m, it.v[i].x, it.v[i].y), b = xform(m, it.v[k].x, it.v[k].y);
+        const float2 vi = ldg2(it.v + i), vk = ldg2(it.v + k);
+        const float2 a = xform(m, vi.x, vi.y), b = xform(m, vk.x, vk.y);
         cnt += pip_edge(px, py, ray_x, a.x, a.y, b.x, b.y);
     }
     return (wave_sum_i32(cnt) & 1) == 1;

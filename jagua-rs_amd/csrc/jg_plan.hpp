@@ -106,14 +106,14 @@ __device__ __forceinline__ int group_sum_i32(int v, uint32_t G) {
 // with the same arguments.
 __device__ __forceinline__ bool pip_hazard_group(const CdeView& c, uint32_t slot, float px, float py, uint32_t sub,
                                                  uint32_t G) {
-    const float4 hb = c.haz_bbox[slot];
+    const float4 hb = ldg4(c.haz_bbox + slot);
     if (!box_has_point(hb, px, py)) return false;
     const float ray_x = hb.z + (hb.z - hb.x);
-    const uint2 vr = c.haz_vr[slot];
+    const uint2 vr = ldg2u(c.haz_vr + slot);
     const float2* v = c.haz_verts + vr.x;
     int cnt = 0;
     for (uint32_t i = sub; i < vr.y; i += G) {
-        const float2 a = v[i], b = v[i + 1 == vr.y ? 0 : i + 1];
+        const float2 a = ldg2(v + i), b = ldg2(v + (i + 1 == vr.y ? 0 : i + 1));
         cnt += pip_edge(px, py, ray_x, a.x, a.y, b.x, b.y);
     }
     return (group_sum_i32(cnt, G) & 1) == 1;
@@ -125,7 +125,8 @@ __device__ __forceinline__ bool pip_query_group(const ItemView& it, const Mat& m
     int cnt = 0;
     for (uint32_t i = sub; i < it.n_v; i += G) {
         const uint32_t k = i + 1 == it.n_v ? 0 : i + 1;
-        const float2 a = xform(m, it.v[i].x, it.v[i].y), b = xform(m, it.v[k].x, it.v[k].y);
+        const float2 vi = ldg2(it.v + i), vk = ldg2(it.v + k);
+        const float2 a = xform(m, vi.x, vi.y), b = xform(m, vk.x, vk.y);
         cnt += pip_edge(px, py, ray_x, a.x, a.y, b.x, b.y);
     }
     return (group_sum_i32(cnt, G) & 1) == 1;
