@@ -135,6 +135,8 @@ static void set_view(jg_cde* c) {
     v.haz_poi = (const float4*)(b + im.o_hp);
     v.haz_vr = (const uint2*)(b + im.o_hv);
     v.haz_verts = (const float2*)(b + im.o_vv);
+    v.cells = (const uint32_t*)(b + im.o_ce);
+    v.cell_level = im.cell_level;
     v.n_slots = (uint32_t)im.slot_ver.size();
     v.exterior_slot = c->exterior.idx;
     v.cd_threshold = c->cfg.cd_threshold;
@@ -220,6 +222,7 @@ jg_status cde_upload(jg_cde* c) {
         put(im.o_hp, im.hpoi.data(), sizeof(float4) * im.hpoi.size());
         put(im.o_hv, im.hvr.data(), sizeof(uint2) * im.hvr.size());
         put(im.o_vv, im.verts.data(), sizeof(float2) * im.verts.size());
+        put(im.o_ce, im.cells.data(), sizeof(uint32_t) * im.cells.size());
         if (c->dev.ptr) HIP_TRY(hipDeviceSynchronize());   // earlier launches may still read the old image
         s = c->dev.reserve(im.total);
         if (s != JG_OK) return s;

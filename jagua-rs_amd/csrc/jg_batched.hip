@@ -323,16 +323,18 @@ __device__ __forceinline__ void poly_queries(const SetDesc& sd, uint32_t q0, uin
             done = true;
         }
         int vr = 0;
+        uint32_t vr_depth = 0;
         if (G == 64u) {
             // one query per wavefront: its bbox is wave-uniform, so the descent's node reads go
             // through the scalar cache instead of a vector load per level
             const float4 ub = make_float4(uniform_f32(bb.x), uniform_f32(bb.y), uniform_f32(bb.z), uniform_f32(bb.w));
-            if (__builtin_amdgcn_readfirstlane(active ? 1 : 0)) vr = virtual_root(c, ub);
+            if (__builtin_amdgcn_readfirstlane(active ? 1 : 0)) vr = virtual_root(c, ub, vr_depth);
         } else if (active) {
-            vr = virtual_root(c, bb);
+            vr = virtual_root(c, bb, vr_depth);
         }
         // edges of the pass's unfinished queries, descending from their virtual roots
-        // (cd_engine.rs:170-175); a query slot is marked hit as soon as one edge collides
+        // (cd_engine.rs:170-175), each from the deepest node below its virtual root known to
+        // hold it (start_node); a query slot is marked hit as soon as one edge collides
         wave_sync();
         if (sub == 0) {
             W.hit[slot] = done ? 1u : 0u;
@@ -348,6 +350,7 @@ __device__ __forceinline__ void poly_queries(const SetDesc& sd, uint32_t q0, uin
                 const uint32_t i = i0 + j * G + sub;
                 const uint32_t o = j * WAVE + lane;
                 const bool want = active && i < nv && W.hit[slot] == 0u;
+                uint32_t start = (uint32_t)vr;
                 if (want) {
                     const uint32_t k = i + 1 == nv ? 0 : i + 1;
                     const float2 vi = ldg2(it.v + i), vk = ldg2(it.v + k);
@@ -355,8 +358,13 @@ __device__ __forceinline__ void poly_queries(const SetDesc& sd, uint32_t q0, uin
                     const float2 b2 = xform(m, vk.x, vk.y);
                     W.geom[o] = make_float4(a.x, a.y, b2.x, b2.y);
                     W.ent_slot[o] = (uint8_t)slot;
+                    const float4 eb = make_float4(mn(a.x, b2.x), mn(a.y, b2.y), mx(a.x, b2.x), mx(a.y, b2.y));
+                    // both hold the edge, so one is on the other's root path: the deeper one
+                    uint32_t depth;
+                    const uint32_t sn = start_node(c, root_bb, eb, 0.0f, depth);
+                    if (depth > vr_depth) start = sn;
                 }
-                n = wave_push_roots(W, n, want, o, (uint32_t)vr, lane);
+                n = wave_push_roots(W, n, want, o, start, lane);
             }
             wave_traverse<FILT, KIND_EDGE, 1, POLY_EG>(c, f, W, n, lane);
         }

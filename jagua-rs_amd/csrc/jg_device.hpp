@@ -88,7 +88,10 @@ struct CdeView {
     const float4* haz_poi;            // cx, cy, r, interior
     const uint2* haz_vr;
     const float2* haz_verts;
+    const uint32_t* cells;            // the deepest node on the path to each cell of a
+                                      // 2^cell_level grid over the root box (row iy, column ix)
     uint32_t n_slots, exterior_slot, cd_threshold, n_nodes;
+    uint32_t cell_level, pad_[3];
     float4 ext_inner;                 // a box inside the container (the exterior hazard's shape):
                                       // its bbox when that shape is an axis-aligned rectangle,
                                       // else empty (x_min > x_max)
@@ -175,16 +178,58 @@ __device__ __forceinline__ bool descend(const CdeView& c, int start, Visit visit
 }
 
 // CDEngine::get_virtual_root (cd_engine.rs:314-327); children's boxes are the quadrants of
-// the node's box (rect.rs:141-151), exactly as the tree built them.
-__device__ __forceinline__ int virtual_root(const CdeView& c, float4 bb) {
+// the node's box (rect.rs:141-151), exactly as the tree built them.  depth: the node's depth.
+__device__ __forceinline__ int virtual_root(const CdeView& c, float4 bb, uint32_t& depth) {
     int n = 0;
+    depth = 0;
     for (;;) {
         const DevNode nd = ldg_node(c.nodes + n);
         if (nd.first_child < 0) return n;
         const int k = quadrant_surrounding(nd.bbox, bb);
         if (k < 0) return n;
         n = nd.first_child + k;
+        depth++;
     }
+}
+__device__ __forceinline__ int virtual_root(const CdeView& c, float4 bb) {
+    uint32_t depth;
+    return virtual_root(c, bb, depth);
+}
+
+// A node from which the descent of an entity with bounding box eb may start instead of the
+// root, and its depth: following the root box's quadrant subdivision (box_quadrant's
+// midpoints, the boxes the tree's nodes have) while eb lies strictly inside one quadrant, by at
+// least m, down to the cell tables' deepest level, the cell table gives the deepest node on
+// that path.  No memory is read until that one entry.
+//
+// QTNode::collides from the root visits this node with the same outcome.  Every hazard present
+// below a node is an entry of it (qt_node.rs:39-81: hazards reach children by constrict, and
+// an Entire hazard is Entire in every quadrant, qt_hazard.rs:58), and Entire entries come first
+// (qt_hazard_vec.rs), so along the root path: an ancestor without relevant hazards leaves this
+// node none either, and an ancestor whose strongest relevant hazard is Entire leaves it Entire
+// here too.  An entity strictly inside each quadrant of the path collides with no quadrant off
+// the path at any ancestor: for an edge, the box test of collides_with_quadrants finds those
+// quadrants disjoint (qt_traits.rs:19-106); for a circle, the margin m exceeds the rounding of
+// circle_box's distance (circle.rs:115-127).  So the ancestors' visits reduce to this one.
+constexpr uint32_t CELL_NODE = 0xffffffu;    // a cell table entry: depth << 24 | node
+__device__ __forceinline__ uint32_t start_node(const CdeView& c, float4 root, float4 eb, float m, uint32_t& depth) {
+    depth = 0;
+    if (c.cell_level == 0u) return 0u;
+    if (!(eb.x - m > root.x && eb.z + m < root.z && eb.y - m > root.y && eb.w + m < root.w)) return 0u;
+    float4 r = root;
+    uint32_t ix = 0, iy = 0, l = 0;
+    for (; l < c.cell_level; l++) {
+        const float mxm = (r.x + r.z) / 2.0f, mym = (r.y + r.w) / 2.0f;
+        const bool xh = eb.x - m > mxm, xl = eb.z + m < mxm, yh = eb.y - m > mym, yl = eb.w + m < mym;
+        if (!(xh || xl) || !(yh || yl)) break;
+        r = make_float4(xh ? mxm : r.x, yh ? mym : r.y, xh ? r.z : mxm, yh ? r.w : mym);
+        ix = 2u * ix + (xh ? 1u : 0u);
+        iy = 2u * iy + (yh ? 1u : 0u);
+    }
+    if (l == 0u) return 0u;
+    const uint32_t e = ldg1(c.cells + ((1u << (2u * l)) - 1u) / 3u + ((iy << l) | ix));
+    depth = e >> 24;
+    return e & CELL_NODE;
 }
 
 // QTNode::collides for a circle (qt_node.rs:99-147; Circle quadrant test qt_traits.rs:10-13)

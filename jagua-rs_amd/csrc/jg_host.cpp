@@ -717,6 +717,36 @@ static void image_write_slot(PoolImage& im, const SlotMap& map, uint32_t s) {
 
 static uint64_t with_headroom(uint64_t n) { return n + n / 2 + 1024; }
 
+namespace {
+// Writes depth << 24 | n for node n (at depth `depth`, covering cell (ix, iy) of the 2^level
+// grid, side cells per side at that level) into the tables of levels level .. cell_level.
+void fill_cells(PoolImage& im, const QuadTree& qt, int32_t n, uint32_t depth, uint32_t level, uint32_t ix, uint32_t iy) {
+    const int32_t ch = qt.pool[(size_t)n].child;
+    const uint32_t entry = (depth << 24) | (uint32_t)n;
+    for (uint32_t l = level, s = 1; l <= im.cell_level; l++, s *= 2) {
+        // the block of cells of level l under this level-`level` cell
+        const size_t off = (((size_t)1 << (2 * l)) - 1) / 3;
+        for (uint32_t y = iy * s; y < (iy + 1) * s; y++)
+            for (uint32_t x = ix * s; x < (ix + 1) * s; x++) im.cells[off + (((size_t)y << l) | x)] = entry;
+        if (ch >= 0 && l == level && level < im.cell_level) break;     // the children write deeper levels
+    }
+    if (ch < 0 || level == im.cell_level) return;
+    fill_cells(im, qt, ch + 0, depth + 1, level + 1, 2 * ix + 1, 2 * iy + 1);
+    fill_cells(im, qt, ch + 1, depth + 1, level + 1, 2 * ix, 2 * iy + 1);
+    fill_cells(im, qt, ch + 2, depth + 1, level + 1, 2 * ix, 2 * iy);
+    fill_cells(im, qt, ch + 3, depth + 1, level + 1, 2 * ix + 1, 2 * iy);
+}
+}  // namespace
+
+void image_cells(PoolImage& im, const QuadTree& qt) {
+    const float4 r = qt.pool[0].bbox;
+    // a degenerate root box has no usable grid: level 0 only, the root
+    const bool flat = !(r.z > r.x && r.w > r.y);
+    im.cell_level = flat ? 0u : std::min<uint32_t>(qt.pool[0].level, CELL_LEVELS_MAX);
+    im.cells.assign((((size_t)1 << (2 * (im.cell_level + 1))) - 1) / 3, 0u);
+    fill_cells(im, qt, 0, 0, 0, 0, 0);
+}
+
 void image_build(PoolImage& im, QuadTree& qt, const SlotMap& map) {
     qt.compact_bfs();
     const uint32_t N = (uint32_t)qt.pool.size();
@@ -737,6 +767,7 @@ void image_build(PoolImage& im, QuadTree& qt, const SlotMap& map) {
     im.slot_ver.assign(S, 0u);
     for (uint32_t s = 1; s < S; s++)
         if (map.slots[s].occ) image_write_slot(im, map, s);
+    image_cells(im, qt);
     im.cap_nodes = with_headroom(N);
     im.cap_ent = with_headroom(im.ent.size());
     im.cap_edges = with_headroom(im.edges.size());
@@ -756,6 +787,7 @@ void image_build(PoolImage& im, QuadTree& qt, const SlotMap& map) {
     im.o_hp = place(sizeof(float4) * im.cap_slots);
     im.o_hv = place(sizeof(uint2) * im.cap_slots);
     im.o_vv = place(sizeof(float2) * im.cap_verts);
+    im.o_ce = place(sizeof(uint32_t) * im.cells.size());
     im.total = off;
     im.valid = true;
     qt.touched.clear();
@@ -826,6 +858,12 @@ bool image_patch(PoolImage& im, QuadTree& qt, const SlotMap& map, std::vector<ui
     }
     if (im.verts.size() > verts0)
         add(im.o_vv + sizeof(float2) * (uint64_t)verts0, im.verts.data() + verts0, sizeof(float2) * (im.verts.size() - verts0));
+    if (!nodes.empty()) {
+        // children made or dropped near the root change the cell table
+        const std::vector<uint32_t> before = im.cells;
+        image_cells(im, qt);
+        if (im.cells != before) add(im.o_ce, im.cells.data(), sizeof(uint32_t) * im.cells.size());
+    }
     qt.touched.clear();
     return true;
 }

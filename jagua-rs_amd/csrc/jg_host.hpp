@@ -165,12 +165,23 @@ struct PoolImage {
     std::vector<uint2> hvr;
     std::vector<float2> verts;
     std::vector<uint32_t> slot_ver;       // version written per slot, 0 = none
+    // cell tables of levels 0 .. cell_level, one after another (level l at (4^l - 1) / 3):
+    // entry (ix, iy) of level l (at iy << l | ix) is depth << 24 | node for the deepest node on
+    // the tree path to cell (ix, iy) of the 2^l grid of the root box's quadrant subdivision
+    std::vector<uint32_t> cells;
+    uint32_t cell_level = 0;
     uint64_t garbage = 0;                 // entry + edge records no node refers to
     bool building = false;                // inside image_build: ranges are packed tightly
     // device layout: record capacities and byte offsets
     uint64_t cap_nodes = 0, cap_ent = 0, cap_edges = 0, cap_slots = 0, cap_verts = 0;
-    uint64_t o_nd = 0, o_nu = 0, o_en = 0, o_ed = 0, o_hb = 0, o_hp = 0, o_hv = 0, o_vv = 0, total = 0;
+    uint64_t o_nd = 0, o_nu = 0, o_en = 0, o_ed = 0, o_hb = 0, o_hp = 0, o_hv = 0, o_vv = 0, o_ce = 0, total = 0;
 };
+
+// Levels of the cell tables: the tree's depth, at most this many (levels 0 .. 7: 87 KB).
+constexpr uint32_t CELL_LEVELS_MAX = 7;
+// Fills im.cells from the tree (quadrant k of a node: 0 = x high y high, 1 = x low y high,
+// 2 = x low y low, 3 = x high y low, as box_quadrant orders them).
+void image_cells(PoolImage& im, const QuadTree& qt);
 
 // Full build of the host mirror and the device layout (capacities with headroom).
 void image_build(PoolImage& im, QuadTree& qt, const SlotMap& map);

@@ -78,6 +78,7 @@ static void full_copy(const PoolImage& im, std::vector<uint8_t>& dev) {
     put(im.o_hp, im.hpoi.data(), sizeof(float4) * im.hpoi.size());
     put(im.o_hv, im.hvr.data(), sizeof(uint2) * im.hvr.size());
     put(im.o_vv, im.verts.data(), sizeof(float2) * im.verts.size());
+    put(im.o_ce, im.cells.data(), sizeof(uint32_t) * im.cells.size());
 }
 
 static void verify(const PoolImage& im, const std::vector<uint8_t>& dev, const QuadTree& qt, const SlotMap& map) {
@@ -116,6 +117,24 @@ static void verify(const PoolImage& im, const std::vector<uint8_t>& dev, const Q
         }
     }
     CHECK(i == f.nodes.size(), "reachable nodes %zu vs %zu", i, f.nodes.size());
+    // the cell tables: entry (ix, iy) of level l is depth << 24 | node for the deepest node on the
+    // tree path to that cell
+    const uint32_t L = im.cell_level;
+    CHECK(L == (qt.pool[0].level < CELL_LEVELS_MAX ? qt.pool[0].level : CELL_LEVELS_MAX), "cell level %u", L);
+    for (uint32_t l = 0; l <= L; l++) {
+        const uint64_t off = ((1ull << (2 * l)) - 1) / 3;
+        for (uint32_t iy = 0; iy < (1u << l); iy++)
+            for (uint32_t ix = 0; ix < (1u << l); ix++) {
+                int32_t n = 0;
+                uint32_t d = 0;
+                for (; d < l && qt.pool[(size_t)n].child >= 0; d++) {
+                    const uint32_t bx = (ix >> (l - 1u - d)) & 1u, by = (iy >> (l - 1u - d)) & 1u;
+                    n = qt.pool[(size_t)n].child + (bx ? (by ? 0 : 3) : (by ? 1 : 2));
+                }
+                CHECK(rd<uint32_t>(dev, im.o_ce + 4ull * (off + (((uint64_t)iy << l) | ix))) == ((d << 24) | (uint32_t)n),
+                      "level %u cell (%u, %u)", l, ix, iy);
+            }
+    }
     CHECK(im.slot_ver.size() == map.slots.size(), "slot count");
     for (size_t s = 1; s < map.slots.size(); s++) {
         if (!map.slots[s].occ) continue;
