@@ -565,7 +565,7 @@ __global__ __launch_bounds__(64) void k_collect(CdeView c, ItemView it, const fl
 // than it keeps, counted at counters[C_OVERFLOW]): (set << 32 | query in set) entries of ovf,
 // each collected here by one wave with the per-query collector over every slot.
 __global__ __launch_bounds__(64) void k_collect_overflow(const SetDesc* __restrict__ sets,
-                                                         const uint32_t* __restrict__ counters,
+                                                         uint32_t* __restrict__ counters,
                                                          const uint64_t* __restrict__ ovf, const float* __restrict__ m6,
                                                          uint32_t words, uint32_t cap, uint32_t* __restrict__ count,
                                                          uint64_t* __restrict__ hash, uint32_t* __restrict__ keys) {
@@ -585,10 +585,15 @@ __global__ __launch_bounds__(64) void k_collect_overflow(const SetDesc* __restri
         collect_query(sd.cde, sd.item, load_mat(m6, out), out, sd.seed, sd.n_seed, sw, cap, lds, list, &list_n, count,
                       hash, keys, lane);
     }
+    // the collection's last launch: its last wave leaves every claim counter at zero, as the
+    // table's other launch sequences expect
+    if (last_wave_out(counters, lane))
+        for (uint32_t i = lane; i < N_COUNTER_WORDS; i += 64u)
+            if (i != C_DONE) counters[i] = 0u;
 }
 
 namespace jg {
-void collect_overflow_launch(const SetDesc* sets, const uint32_t* counters, const uint64_t* ovf, const float* m6,
+void collect_overflow_launch(const SetDesc* sets, uint32_t* counters, const uint64_t* ovf, const float* m6,
                              uint32_t words, uint32_t cap, uint32_t* count, uint64_t* hash, uint32_t* keys,
                              uint32_t grid, hipStream_t stream) {
     hipLaunchKernelGGL(k_collect_overflow, dim3(grid), dim3(64), (size_t)words * 4, stream, sets, counters, ovf, m6,

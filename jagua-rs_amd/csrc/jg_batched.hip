@@ -49,10 +49,6 @@ __device__ __forceinline__ WorkList wave_list(WaveLds& L) {
 
 }  // namespace
 
-__global__ void k_reset_counters(uint32_t* counters) {
-    for (uint32_t i = threadIdx.x; i < N_COUNTER_WORDS; i += blockDim.x) counters[i] = 0u;
-}
-
 // Hazard edges per edge-test item of the surrogate stage's traversals (pole circles and piers)
 // and of the polygon stage's (query edges).
 constexpr int SURR_EG = 4;
@@ -130,6 +126,8 @@ __global__ __launch_bounds__(WAVE) void k_surrogate(const SetDesc* __restrict__ 
     const uint32_t lane = threadIdx.x;
     __shared__ WaveLds lds;
     const WorkList W = wave_list(lds);
+    // the polygon stage of this launch claims groups after this kernel: clear its claim counters
+    if (blockIdx.x == 0 && lane < N_STREAMS) counters[C_STREAM0 + STREAM_STRIDE * lane] = 0u;
     uint32_t ux = blockIdx.x & (N_STREAMS - 1u), utried = 0;
     for (;;) {
         const uint32_t u = claim_next(counters, C_USTREAM0, ux, utried, n_units, lane);
@@ -328,9 +326,9 @@ __device__ __forceinline__ void poly_queries(const SetDesc& sd, uint32_t q0, uin
             // one query per wavefront: its bbox is wave-uniform, so the descent's node reads go
             // through the scalar cache instead of a vector load per level
             const float4 ub = make_float4(uniform_f32(bb.x), uniform_f32(bb.y), uniform_f32(bb.z), uniform_f32(bb.w));
-            if (__builtin_amdgcn_readfirstlane(active ? 1 : 0)) vr = virtual_root(c, ub, vr_depth);
+            if (__builtin_amdgcn_readfirstlane(active ? 1 : 0)) vr = virtual_root_cells(c, root_bb, ub, vr_depth);
         } else if (active) {
-            vr = virtual_root(c, bb, vr_depth);
+            vr = virtual_root_cells(c, root_bb, bb, vr_depth);
         }
         // edges of the pass's unfinished queries, descending from their virtual roots
         // (cd_engine.rs:170-175), each from the deepest node below its virtual root known to
@@ -420,6 +418,8 @@ __device__ __forceinline__ void poly_loop(const SetDesc* __restrict__ sets, uint
                                           const float* __restrict__ m6, uint8_t* __restrict__ status, WaveLds& lds) {
     const uint32_t lane = threadIdx.x;
     const WorkList W = wave_list(lds);
+    // the surrogate stage of this launch is done: clear its unit-claim counters for the next
+    if (blockIdx.x == 0 && lane < N_STREAMS) counters[C_USTREAM0 + STREAM_STRIDE * lane] = 0u;
     // stream x holds the groups its units appended; a wave drains its own stream, then the next
     uint32_t x = blockIdx.x & (N_STREAMS - 1u);
     for (uint32_t tried = 0; tried < N_STREAMS;) {
@@ -435,6 +435,8 @@ __device__ __forceinline__ void poly_loop(const SetDesc* __restrict__ sets, uint
         const Group gr = groups[gbase[x] + j];
         poly_queries<FILT>(sets[gr.set], gr.q0, gr.mask, W, m6, status, lane);
     }
+    // every wave has read the group counts once the last one is out
+    if (last_wave_out(counters, lane) && lane < N_STREAMS) counters[C_GCOUNT0 + STREAM_STRIDE * lane] = 0u;
 }
 
 // The filtered instances carry the filter state on top and would need a few VGPRs over the 128
@@ -472,13 +474,15 @@ __device__ __forceinline__ void fused_loop(const SetDesc* __restrict__ sets, con
     uint32_t ux = blockIdx.x & (N_STREAMS - 1u), utried = 0;
     for (;;) {
         const uint32_t u = claim_next(counters, C_USTREAM0, ux, utried, n_units, lane);
-        if (u >= n_units) return;
+        if (u >= n_units) break;
         const Unit un = units[u];
         const SetDesc& sd = sets[un.set];
         const uint64_t todo = surrogate_unit<FILT, KIND, FUSED_IPL>(sd, un, W, m6, status, lane);
         if (todo != 0ull) poly_queries<FILT>(sd, un.q0, todo, W, m6, status, lane);
         wave_sync();
     }
+    // every wave has stopped claiming once the last one is out
+    if (last_wave_out(counters, lane) && lane < N_STREAMS) counters[C_USTREAM0 + STREAM_STRIDE * lane] = 0u;
 }
 
 template <int KIND>
@@ -624,14 +628,14 @@ jg_status table_image(const std::vector<jg_query_set>& sets, const std::vector<I
     return JG_OK;
 }
 
-// The launch sequence over a table at device address `base`: the counter reset (a kernel, so
-// that the sequence can be captured in a graph), then the two stages or the fused kernel.
+// The launch sequence over a table at device address `base`: the two stages or the fused
+// kernel.  Their claim counters start at zero (a new table's are; each sequence, detection or
+// collection, leaves them so), so no clearing step precedes them.
 jg_status table_launch(const TableLayout& L, uint8_t* base, const float* d_m6, uint8_t* d_status,
                        hipStream_t stream) {
     uint32_t* counters = (uint32_t*)(base + L.o_ctr);
     Group* groups = (Group*)(base + L.o_grp);
     const uint32_t* gbase = (const uint32_t*)(base + L.o_gbase);
-    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream, counters);
     const SetDesc* sets = (const SetDesc*)base;
     const Unit* units = (const Unit*)(base + L.o_units);
     const dim3 gs(L.grid_s), gp(L.grid_p), blk(WAVE);

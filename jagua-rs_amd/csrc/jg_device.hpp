@@ -97,6 +97,8 @@ struct CdeView {
                                       // else empty (x_min > x_max)
 };
 
+constexpr uint32_t CELL_NODE = 0xffffffu;    // a cell table entry: depth << 24 | node
+
 struct ItemView {
     const float2* v;                  // local vertices (CCW)
     const float4* ff;                 // ff poles (cx, cy, r, 0) then piers (x1 y1 x2 y2)
@@ -196,6 +198,39 @@ __device__ __forceinline__ int virtual_root(const CdeView& c, float4 bb) {
     return virtual_root(c, bb, depth);
 }
 
+// virtual_root through the cell tables: the quadrants of the root box (root: node 0's box) are
+// followed arithmetically (box_quadrant, the first quadrant that surrounds bb, as the search
+// takes it) down to the tables' deepest level or the first level where none surrounds bb; the
+// table entry of that cell is the deepest node on the path, where the search ends unless it
+// lies at the tables' deepest level, from which it goes on through node records.
+__device__ __forceinline__ int virtual_root_cells(const CdeView& c, float4 root, float4 bb, uint32_t& depth) {
+    int n = 0;
+    depth = 0;
+    if (c.cell_level > 0u) {
+        float4 r = root;
+        uint32_t ix = 0, iy = 0, l = 0;
+        for (; l < c.cell_level; l++) {
+            const int k = quadrant_surrounding(r, bb);
+            if (k < 0) break;
+            r = box_quadrant(r, k);
+            ix = 2u * ix + ((k == 0 || k == 3) ? 1u : 0u);
+            iy = 2u * iy + ((k == 0 || k == 1) ? 1u : 0u);
+        }
+        const uint32_t e = ldg1(c.cells + ((1u << (2u * l)) - 1u) / 3u + ((iy << l) | ix));
+        n = (int)(e & CELL_NODE);
+        depth = e >> 24;
+        if (depth < l || l < c.cell_level) return n;
+    }
+    for (;;) {
+        const DevNode nd = ldg_node(c.nodes + n);
+        if (nd.first_child < 0) return n;
+        const int k = quadrant_surrounding(nd.bbox, bb);
+        if (k < 0) return n;
+        n = nd.first_child + k;
+        depth++;
+    }
+}
+
 // A node from which the descent of an entity with bounding box eb may start instead of the
 // root, and its depth: following the root box's quadrant subdivision (box_quadrant's
 // midpoints, the boxes the tree's nodes have) while eb lies strictly inside one quadrant, by at
@@ -211,7 +246,6 @@ __device__ __forceinline__ int virtual_root(const CdeView& c, float4 bb) {
 // the path at any ancestor: for an edge, the box test of collides_with_quadrants finds those
 // quadrants disjoint (qt_traits.rs:19-106); for a circle, the margin m exceeds the rounding of
 // circle_box's distance (circle.rs:115-127).  So the ancestors' visits reduce to this one.
-constexpr uint32_t CELL_NODE = 0xffffffu;    // a cell table entry: depth << 24 | node
 __device__ __forceinline__ uint32_t start_node(const CdeView& c, float4 root, float4 eb, float m, uint32_t& depth) {
     depth = 0;
     if (c.cell_level == 0u) return 0u;

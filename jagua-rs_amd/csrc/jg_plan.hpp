@@ -42,13 +42,27 @@ struct Group {             // 16 B: surviving queries q0 + i for the set bits i 
 //                   the group array (base gbase[x]); C_GCOUNT0 + 32 x counts the groups
 //                   appended, C_STREAM0 + 32 x the groups taken
 // A wave starts on stream blockIdx.x mod 8 and moves to the next one as each runs dry.
-// C_OVERFLOW counts the queries the batched collection hands to the per-query collector.
+// C_OVERFLOW counts the queries the batched collection hands to the per-query collector;
+// C_DONE counts the waves of a launch that have finished (the last one clears counters).
+// Every launch sequence leaves all counters at zero for the next, and a new table starts so.
 constexpr uint32_t N_STREAMS = 8, STREAM_STRIDE = 32;
 constexpr uint32_t C_STREAM0 = 0;
 constexpr uint32_t C_USTREAM0 = C_STREAM0 + STREAM_STRIDE * N_STREAMS;
 constexpr uint32_t C_GCOUNT0 = C_USTREAM0 + STREAM_STRIDE * N_STREAMS;
 constexpr uint32_t C_OVERFLOW = C_GCOUNT0 + STREAM_STRIDE * N_STREAMS;
-constexpr uint32_t N_COUNTER_WORDS = C_OVERFLOW + STREAM_STRIDE;
+constexpr uint32_t C_DONE = C_OVERFLOW + STREAM_STRIDE;
+constexpr uint32_t N_COUNTER_WORDS = C_DONE + STREAM_STRIDE;
+
+// The calling wave's exit from a launch of gridDim.x one-wave workgroups: true in the last
+// wave to leave, once every other wave has left (C_DONE is then back at zero).
+__device__ __forceinline__ bool last_wave_out(uint32_t* counters, uint32_t lane) {
+    uint32_t last = 0;
+    if (lane == 0) {
+        last = atomicAdd(&counters[C_DONE], 1u) == gridDim.x - 1u ? 1u : 0u;
+        if (last) counters[C_DONE] = 0u;
+    }
+    return __builtin_amdgcn_readfirstlane(last) != 0u;
+}
 
 constexpr uint32_t WAVE = 64;
 
@@ -139,7 +153,7 @@ jg_status collect_launch(const TableLayout& L, uint8_t* base, const float* d_m6,
                          uint64_t* d_hash, uint32_t* d_keys, uint64_t* ovf, uint32_t n_cus, hipStream_t stream);
 
 // The per-query collector for the queries the batched collection handed over (jg_api.hip).
-void collect_overflow_launch(const SetDesc* sets, const uint32_t* counters, const uint64_t* ovf, const float* m6,
+void collect_overflow_launch(const SetDesc* sets, uint32_t* counters, const uint64_t* ovf, const float* m6,
                              uint32_t words, uint32_t cap, uint32_t* count, uint64_t* hash, uint32_t* keys,
                              uint32_t grid, hipStream_t stream);
 
