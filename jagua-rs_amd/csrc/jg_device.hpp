@@ -535,6 +535,8 @@ __device__ __forceinline__ void wave_traverse(const CdeView& c, const FilterView
         }
         const uint32_t base = nn - take;
         uint32_t item[IPL], m[IPL], n_edges[IPL], slot[IPL];
+        uint32_t p_beg[IPL], p_cnt[IPL], n_pass[IPL];   // a leaf's first passing entry's edges; passing entries
+                                                        // (kept with one item per lane)
         int fc[IPL], s[IPL], he[IPL];
         bool circle[IPL];
         float4 g[IPL];
@@ -552,6 +554,11 @@ __device__ __forceinline__ void wave_traverse(const CdeView& c, const FilterView
             const uint32_t o = item[r] >> 24;
             m[r] = 0;
             n_edges[r] = 0;
+            if constexpr (IPL == 1) {
+                p_beg[r] = 0;
+                p_cnt[r] = 0;
+                n_pass[r] = 0;
+            }
             fc[r] = 0;
             s[r] = -1;
             he[r] = 0;
@@ -579,10 +586,24 @@ __device__ __forceinline__ void wave_traverse(const CdeView& c, const FilterView
                     } else if (s[r] >= 0) {
                         // leaf: count the edge tests of its relevant Partial hazards
                         he[r] = (int)(nd[r].haz_begin + (nd[r].haz_info & COUNT_MASK));
-                        for (int i = s[r]; i < he[r]; i++) {
-                            const DevEntry e = ldg_ent(c.ent + i);
-                            if (!irrel<FILT>(f, q[r], e.slot_flags & SLOT_MASK) && ff_pass(e, circle[r], g[r]))
-                                n_edges[r] += (e.edge_count + (uint32_t)EG - 1u) / (uint32_t)EG;
+                        if constexpr (IPL == 1) {
+                            for (int i = s[r]; i < he[r]; i++) {
+                                const DevEntry e = ldg_ent(c.ent + i);
+                                if (!irrel<FILT>(f, q[r], e.slot_flags & SLOT_MASK) && ff_pass(e, circle[r], g[r])) {
+                                    if (n_pass[r] == 0u) {
+                                        p_beg[r] = e.edge_begin;
+                                        p_cnt[r] = e.edge_count;
+                                    }
+                                    n_pass[r]++;
+                                    n_edges[r] += (e.edge_count + (uint32_t)EG - 1u) / (uint32_t)EG;
+                                }
+                            }
+                        } else {
+                            for (int i = s[r]; i < he[r]; i++) {
+                                const DevEntry e = ldg_ent(c.ent + i);
+                                if (!irrel<FILT>(f, q[r], e.slot_flags & SLOT_MASK) && ff_pass(e, circle[r], g[r]))
+                                    n_edges[r] += (e.edge_count + (uint32_t)EG - 1u) / (uint32_t)EG;
+                            }
                         }
                     }
                 }
@@ -620,7 +641,16 @@ __device__ __forceinline__ void wave_traverse(const CdeView& c, const FilterView
 #pragma unroll
         for (int r = 0; r < IPL; r++) {
             if (!n_edges[r]) continue;
-            if (fits) {
+            if (IPL == 1 && fits && n_pass[r] == 1u) {
+                // one passing entry (the common leaf): its edges from the count above (measured
+                // faster with one item per lane; with two, the extra registers cost more)
+                const uint32_t o = item[r] >> 24;
+                for (uint32_t j = 0; j < p_cnt[r]; j += EG, at++) {
+                    const uint32_t k = p_cnt[r] - j < (uint32_t)EG ? p_cnt[r] - j : (uint32_t)EG;
+                    W.edges[at] = (p_beg[r] + j) | ((k - 1u) << 29);
+                    W.edge_ent[at] = (uint8_t)o;
+                }
+            } else if (fits) {
                 const uint32_t o = item[r] >> 24;
                 for (int i = s[r]; i < he[r]; i++) {
                     const DevEntry e = ldg_ent(c.ent + i);
