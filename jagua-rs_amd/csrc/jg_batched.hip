@@ -303,10 +303,16 @@ __device__ __forceinline__ void poly_queries(const SetDesc& sd, uint32_t q0, uin
         // transformed bbox (simple_polygon.rs:122-133), reduced over the group's lanes
         float4 bb = make_float4(3.40282347e+38f, 3.40282347e+38f, -3.40282347e+38f, -3.40282347e+38f);
         if (active)
-            for (uint32_t i = sub; i < nv; i += G) {
-                const float2 vi = ldg2(it.v + i);
-                const float2 p = xform(m, vi.x, vi.y);
-                bb.x = mn(bb.x, p.x); bb.y = mn(bb.y, p.y); bb.z = mx(bb.z, p.x); bb.w = mx(bb.w, p.y);
+            for (uint32_t i = sub; i < nv; i += 4u * G) {
+                // four vertices in flight (past the end: vertex i again, which leaves the box as is)
+                float2 v4[4];
+#pragma unroll
+                for (uint32_t u = 0; u < 4u; u++) v4[u] = ldg2(it.v + (i + u * G < nv ? i + u * G : i));
+#pragma unroll
+                for (uint32_t u = 0; u < 4u; u++) {
+                    const float2 p = xform(m, v4[u].x, v4[u].y);
+                    bb.x = mn(bb.x, p.x); bb.y = mn(bb.y, p.y); bb.z = mx(bb.z, p.x); bb.w = mx(bb.w, p.y);
+                }
             }
         for (uint32_t o = G >> 1; o > 0; o >>= 1) {
             bb.x = fminf(bb.x, __shfl_xor(bb.x, (int)o, 64));
