@@ -135,7 +135,7 @@ __device__ __forceinline__ bool box_inside(float4 in, float4 out) {
 // Whether an entity with box eb may collect hazard `slot` anywhere below a node that holds it.
 __device__ __forceinline__ bool may_collect(const CdeView& c, uint32_t slot, float4 eb) {
     if (slot == c.exterior_slot) return !box_inside(eb, inner_shrunk(c));
-    return boxes_meet(eb, widened(c.haz_bbox[slot]));
+    return boxes_meet(eb, widened(ldg4(c.haz_bbox + slot)));
 }
 
 __device__ __forceinline__ float4 entity_box(bool circle, float4 g) {
@@ -155,12 +155,12 @@ __device__ __forceinline__ void test_inline(const CdeView& c, CollectLds& L, con
                                             bool circle, float4 g) {
     const uint32_t he = nd.haz_begin + (nd.haz_info & COUNT_MASK);
     for (uint32_t i = nd.haz_begin; i < he; i++) {
-        const DevEntry e = c.ent[i];
+        const DevEntry e = ldg_ent(c.ent + i);
         if (!tests_edges(e, circle, g)) continue;
         const uint32_t sl = e.slot_flags & SLOT_MASK;
         if (set_has(L, qs, sl)) continue;
         for (uint32_t j = 0; j < e.edge_count; j++)
-            if (edge_hit(circle, g, c.edges[e.edge_begin + j])) {
+            if (edge_hit(circle, g, ldg4(c.edges + e.edge_begin + j))) {
                 set_put(L, qs, sl);
                 break;
             }
@@ -174,10 +174,10 @@ __device__ __forceinline__ bool surrogate_live(const CdeView& c, const CollectLd
     for (uint32_t i0 = 0; i0 < cnt; i0 += 4) {
         uint32_t sl[4];
 #pragma unroll
-        for (uint32_t r = 0; r < 4; r++) sl[r] = c.ent[nd.haz_begin + (i0 + r < cnt ? i0 + r : 0u)].slot_flags & SLOT_MASK;
+        for (uint32_t r = 0; r < 4; r++) sl[r] = ldg_slot_flags(c.ent + nd.haz_begin + (i0 + r < cnt ? i0 + r : 0u)) & SLOT_MASK;
         float4 hb[4];
 #pragma unroll
-        for (uint32_t r = 0; r < 4; r++) hb[r] = c.haz_bbox[sl[r]];
+        for (uint32_t r = 0; r < 4; r++) hb[r] = ldg4(c.haz_bbox + sl[r]);
 #pragma unroll
         for (uint32_t r = 0; r < 4; r++) {
             if (i0 + r >= cnt || set_has(L, qs, sl[r])) continue;
@@ -201,7 +201,7 @@ __device__ __forceinline__ void wave_collect(const CdeView& c, CollectLds& L, ui
                 const uint32_t at = ne - 1u - lane;
                 const uint32_t o = L.edge_ent[at], sl = L.eslot[at];
                 const uint32_t es = L.ent_slot[o], qs = es & 63u;
-                const float4 he = c.edges[L.edges[at]];
+                const float4 he = ldg4(c.edges + L.edges[at]);
                 if (!L.ovf[qs] && !set_has(L, qs, sl) && edge_hit((es & WL_CIRCLE) != 0u, L.geom[o], he))
                     set_put(L, qs, sl);
             }
@@ -225,7 +225,7 @@ __device__ __forceinline__ void wave_collect(const CdeView& c, CollectLds& L, ui
             // a query edge whose candidates are all collected adds nothing below any node
             live = !L.ovf[qs] && (em == SURR_MASK || (em & ~L.cdone[qs]) != 0u);
         }
-        const DevNode nd = c.nodes[live ? (item & WL_NODE_MASK) : 0u];
+        const DevNode nd = ldg_node(c.nodes + (live ? (item & WL_NODE_MASK) : 0u));
         uint32_t m = 0, n_edges = 0;
         int fc = 0;
         bool circle = false;
@@ -244,14 +244,14 @@ __device__ __forceinline__ void wave_collect(const CdeView& c, CollectLds& L, ui
                 } else if (INLINE_EDGES) {
                     // the visit inserts its Entire hazards and tests its Partial hazards' edges itself
                     for (uint32_t i = nd.haz_begin; i < nd.haz_begin + cnt; i++) {
-                        const uint32_t sf = c.ent[i].slot_flags;
+                        const uint32_t sf = ldg_slot_flags(c.ent + i);
                         const uint32_t sl = sf & SLOT_MASK;
                         if ((sf & ENTIRE_BIT) && !set_has(L, qs, sl)) set_put(L, qs, sl);
                     }
                     test_inline(c, L, nd, qs, circle, g);
                 } else {
                     for (uint32_t i = nd.haz_begin; i < nd.haz_begin + cnt; i++) {
-                        const DevEntry e = c.ent[i];
+                        const DevEntry e = ldg_ent(c.ent + i);
                         const uint32_t sl = e.slot_flags & SLOT_MASK;
                         if (e.slot_flags & ENTIRE_BIT) {
                             if (!set_has(L, qs, sl)) set_put(L, qs, sl);
@@ -282,7 +282,7 @@ __device__ __forceinline__ void wave_collect(const CdeView& c, CollectLds& L, ui
             if (fits) {
                 uint32_t at = ne + (excl >> 16);
                 for (uint32_t i = nd.haz_begin; i < nd.haz_begin + cnt; i++) {
-                    const DevEntry e = c.ent[i];
+                    const DevEntry e = ldg_ent(c.ent + i);
                     if (!tests_edges(e, circle, g)) continue;
                     const uint32_t sl = e.slot_flags & SLOT_MASK;
                     for (uint32_t j = 0; j < e.edge_count; j++, at++) {
@@ -324,10 +324,10 @@ __device__ __forceinline__ void containment_group(const CdeView& c, CollectLds& 
         if (rl == REL_SURROUNDING) {
             contained = pip_hazard_group(c, s2, qpoi.x, qpoi.y, sub, CG);
         } else {
-            const float4 hp = c.haz_poi[s2];
+            const float4 hp = ldg4(c.haz_poi + s2);
             contained = pip_query_group(it, m, bb, hp.x, hp.y, sub, CG);
         }
-        if (lane == (uint32_t)l && (c.haz_poi[s2].w != 0.0f ? contained : !contained)) set_put(L, qs, s2);
+        if (lane == (uint32_t)l && (ldg4(c.haz_poi + s2).w != 0.0f ? contained : !contained)) set_put(L, qs, s2);
     }
 }
 
@@ -348,7 +348,8 @@ __device__ __forceinline__ void collect_pass(const SetDesc& sd, uint32_t set_ind
     float4 bb = make_float4(3.40282347e+38f, 3.40282347e+38f, -3.40282347e+38f, -3.40282347e+38f);
     if (active)
         for (uint32_t i = sub; i < it.n_v; i += CG) {
-            const float2 p = xform(m, it.v[i].x, it.v[i].y);
+            const float2 vi = ldg2(it.v + i);
+            const float2 p = xform(m, vi.x, vi.y);
             bb.x = mn(bb.x, p.x); bb.y = mn(bb.y, p.y); bb.z = mx(bb.z, p.x); bb.w = mx(bb.w, p.y);
         }
     for (uint32_t o = CG >> 1; o > 0; o >>= 1) {
@@ -367,7 +368,7 @@ __device__ __forceinline__ void collect_pass(const SetDesc& sd, uint32_t set_ind
     uint32_t vcnt = 0, ve0 = 0;
     if (active) {
         vr = virtual_root(c, bb);
-        const DevNode vn = c.nodes[vr];
+        const DevNode vn = ldg_node(c.nodes + vr);
         ve0 = vn.haz_begin;
         vcnt = vn.haz_info & COUNT_MASK;
     }
@@ -400,11 +401,11 @@ __device__ __forceinline__ void collect_pass(const SetDesc& sd, uint32_t set_ind
                 for (uint32_t step = CP / 2; step > 0; step >>= 1)
                     if (L.vpre[lo + step] <= p) lo += step;
                 qq[r] = lo;
-                sf[r] = p < T ? c.ent[L.vbeg[lo] + (p - L.vpre[lo])].slot_flags : 0u;
+                sf[r] = p < T ? ldg_slot_flags(c.ent + L.vbeg[lo] + (p - L.vpre[lo])) : 0u;
             }
             float4 hb[4];
 #pragma unroll
-            for (uint32_t r = 0; r < 4; r++) hb[r] = c.haz_bbox[sf[r] & SLOT_MASK];
+            for (uint32_t r = 0; r < 4; r++) hb[r] = ldg4(c.haz_bbox + (sf[r] & SLOT_MASK));
 #pragma unroll
             for (uint32_t r = 0; r < 4; r++) {
                 if (p0 + r * WAVE + lane >= T) continue;
@@ -426,8 +427,8 @@ __device__ __forceinline__ void collect_pass(const SetDesc& sd, uint32_t set_ind
     if (active) {
         // a pre-seeded collector, then the exterior when the shape leaves the engine's bbox
         // (cd_engine.rs:262-264); the traversal goes on regardless
-        for (uint32_t i = sub; i < sd.n_seed; i += CG) set_put(L, qs, sd.seed[i]);
-        if (sub == 0 && box_relation(c.nodes[0].bbox, bb) != REL_SURROUNDING) set_put(L, qs, c.exterior_slot);
+        for (uint32_t i = sub; i < sd.n_seed; i += CG) set_put(L, qs, ldg1(sd.seed + i));
+        if (sub == 0 && box_relation(ldg4((const float4*)c.nodes), bb) != REL_SURROUNDING) set_put(L, qs, c.exterior_slot);
     }
     // every query's ff poles / piers from the root and edges from its virtual root.  A cursor
     // walks the entities 64 at a time; those that may add to their query's set take the next
@@ -451,7 +452,7 @@ __device__ __forceinline__ void collect_pass(const SetDesc& sd, uint32_t set_ind
                 if (!L.ovf[qi]) {
                     const Mat mq = load_mat(m6, sd.first + q0 + qi);
                     if (k < n_ff) {
-                        const float4 p = it.ff[k];
+                        const float4 p = ldg4(it.ff + k);
                         circle = k < it.n_poles;
                         if (circle) {
                             const float2 cc = xform(mq, p.x, p.y);
@@ -463,7 +464,8 @@ __device__ __forceinline__ void collect_pass(const SetDesc& sd, uint32_t set_ind
                         want = true;
                     } else {
                         const uint32_t i = k - n_ff, i1 = i + 1 == it.n_v ? 0 : i + 1;
-                        const float2 a = xform(mq, it.v[i].x, it.v[i].y), b = xform(mq, it.v[i1].x, it.v[i1].y);
+                        const float2 va = ldg2(it.v + i), vb = ldg2(it.v + i1);
+                        const float2 a = xform(mq, va.x, va.y), b = xform(mq, vb.x, vb.y);
                         gm = make_float4(a.x, a.y, b.x, b.y);
                         root = (uint32_t)L.vroot[qi];
                         // the candidates this edge's box may collect (all of them when the
@@ -532,13 +534,13 @@ __device__ __forceinline__ void collect_pass(const SetDesc& sd, uint32_t set_ind
                     const uint32_t cf = L.cand[qs * NC + k];
                     sf = (cf & C_SLOT) | ((cf & C_ENTIRE) ? ENTIRE_BIT : 0u);
                 } else {
-                    sf = c.ent[ve0 + k].slot_flags;
+                    sf = ldg_slot_flags(c.ent + ve0 + k);
                 }
                 sl = sf & SLOT_MASK;
                 if (!(sf & ENTIRE_BIT) && !set_has(L, qs, sl)) {
-                    rel = box_almost_relation(c.haz_bbox[sl], bb);
+                    rel = box_almost_relation(ldg4(c.haz_bbox + sl), bb);
                     if (rel == REL_SURROUNDING || rel == REL_ENCLOSED) pend = true;
-                    else if (c.haz_poi[sl].w == 0.0f) set_put(L, qs, sl);
+                    else if (ldg4(c.haz_poi + sl).w == 0.0f) set_put(L, qs, sl);
                 }
             }
             containment_group(c, L, it, m, bb, qpoi, qs, sub, lane, pend, sl, rel);
