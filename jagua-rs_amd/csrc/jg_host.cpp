@@ -423,6 +423,7 @@ int32_t QuadTree::alloc_children(int32_t parent) {
         if (!(c.bbox.x < c.bbox.z && c.bbox.y < c.bbox.w)) c.bbox = c.bbox;  // degenerate boxes are kept as is
     }
     pool[parent].child = base;
+    if ((uint32_t)(pool[0].level - pool[parent].level) < CELL_LEVELS_MAX) near_root_changed = true;
     touched.push_back(parent);
     for (int k = 0; k < 4; k++) touched.push_back(base + k);
     return base;
@@ -435,6 +436,7 @@ void QuadTree::free_subtree_children(int32_t node) {
     for (int k = 0; k < 4; k++) pool[base + k] = QNode{};
     free_blocks.push_back(base);
     pool[node].child = -1;
+    if ((uint32_t)(pool[0].level - pool[node].level) < CELL_LEVELS_MAX) near_root_changed = true;
     touched.push_back(node);
 }
 
@@ -768,6 +770,7 @@ void image_build(PoolImage& im, QuadTree& qt, const SlotMap& map) {
     for (uint32_t s = 1; s < S; s++)
         if (map.slots[s].occ) image_write_slot(im, map, s);
     image_cells(im, qt);
+    qt.near_root_changed = false;
     im.cap_nodes = with_headroom(N);
     im.cap_ent = with_headroom(im.ent.size());
     im.cap_edges = with_headroom(im.edges.size());
@@ -858,13 +861,13 @@ bool image_patch(PoolImage& im, QuadTree& qt, const SlotMap& map, std::vector<ui
     }
     if (im.verts.size() > verts0)
         add(im.o_vv + sizeof(float2) * (uint64_t)verts0, im.verts.data() + verts0, sizeof(float2) * (im.verts.size() - verts0));
-    if (!nodes.empty()) {
-        // children made or dropped near the root change the cell table
-        const std::vector<uint32_t> before = im.cells;
+    if (qt.near_root_changed) {
+        // children made or dropped within the tables' levels change the cell tables
         image_cells(im, qt);
-        if (im.cells != before) add(im.o_ce, im.cells.data(), sizeof(uint32_t) * im.cells.size());
+        add(im.o_ce, im.cells.data(), sizeof(uint32_t) * im.cells.size());
     }
     qt.touched.clear();
+    qt.near_root_changed = false;
     return true;
 }
 

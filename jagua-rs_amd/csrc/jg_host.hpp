@@ -87,6 +87,9 @@ struct QEntry {                          // QTHazard in one node (qt_hazard.rs:1
     std::vector<uint32_t> edges;         // edge indices into the hazard polygon
 };
 
+// Levels of the cell tables: the tree's depth, at most this many (levels 0 .. 7: 87 KB).
+constexpr uint32_t CELL_LEVELS_MAX = 7;
+
 struct QNode {                           // QTNode (qt_node.rs:14-25)
     uint8_t level = 0;
     float4 bbox{};
@@ -99,6 +102,7 @@ struct QuadTree {
     std::vector<QNode> pool;
     std::vector<int32_t> free_blocks;
     std::vector<int32_t> touched;        // nodes whose record, entries or children changed since the last image patch
+    bool near_root_changed = false;      // children made or dropped within the cell tables' levels since then
     uint8_t cd_threshold = 16;
     void init(uint8_t depth, float4 bbox, uint8_t cdt);
     void register_hazard(int32_t node, QEntry e, const SlotMap& map);
@@ -177,8 +181,6 @@ struct PoolImage {
     uint64_t o_nd = 0, o_nu = 0, o_en = 0, o_ed = 0, o_hb = 0, o_hp = 0, o_hv = 0, o_vv = 0, o_ce = 0, total = 0;
 };
 
-// Levels of the cell tables: the tree's depth, at most this many (levels 0 .. 7: 87 KB).
-constexpr uint32_t CELL_LEVELS_MAX = 7;
 // Fills im.cells from the tree (quadrant k of a node: 0 = x high y high, 1 = x low y high,
 // 2 = x low y low, 3 = x high y low, as box_quadrant orders them).
 void image_cells(PoolImage& im, const QuadTree& qt);
