@@ -1,7 +1,8 @@
 // lbf's sampled placement search (lbf/src/opt/search.rs:17-106) resident on the GPU for many
 // independent (engine, item) searches: per round, k_search_sample draws the samples of every
-// search, the batched collision query (jg_batched.hip) evaluates them, and k_search_select
-// keeps each search's best clear sample and tightens its sampling box.  A run is a fixed
+// search, the batched collision query (jg_batched.hip) evaluates them, k_search_cost prices
+// the clear ones, and k_search_select keeps each search's best clear sample and tightens its
+// sampling box.  A run is a fixed
 // sequence of launches on one stream, optionally captured once in a hipGraph.
 #include <hip/hip_runtime.h>
 
@@ -97,35 +98,54 @@ __global__ __launch_bounds__(SAMPLE_THREADS) void k_search_sample(
     o[0] = r.angle; o[1] = tx; o[2] = ty;
 }
 
-// One wavefront per search: the clear sample of lowest LBFLoss (loss.rs:29-41), ties to the
+// The LBFLoss (loss.rs:29-41) of every clear sample of an active search, one thread per
+// sample: x_max * 10 + y_max of its transformed shape's bbox (simple_polygon.rs:122-133,
+// 226-253), with x_max and y_max; INFINITY for the others.  Samples of one search are
+// consecutive, so a wave's vertex loops mostly share their length.
+__global__ __launch_bounds__(SAMPLE_THREADS) void k_search_cost(const SearchDesc* __restrict__ d,
+                                                               const SearchState* __restrict__ st, uint32_t n,
+                                                               uint32_t K, uint32_t local, const float* __restrict__ m6,
+                                                               const uint8_t* __restrict__ status,
+                                                               float4* __restrict__ cost) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)n * K) return;
+    const uint32_t s = (uint32_t)(i / K);
+    float4 out = make_float4(INFINITY, 0.0f, 0.0f, 0.0f);
+    if ((!local || st[s].found) && status[i] == JG_Q_CLEAR) {
+        const Mat m = load_mat(m6, i);
+        const float2* v = d[s].v;
+        const uint32_t nv = d[s].n_v;
+        float xm = -3.40282347e+38f, ym = -3.40282347e+38f;
+        for (uint32_t k = 0; k < nv; k++) {
+            const float2 vk = ldg2(v + k);
+            const float2 p = xform(m, vk.x, vk.y);
+            xm = mx(xm, p.x);
+            ym = mx(ym, p.y);
+        }
+        out = make_float4(xm * 10.0f + ym, xm, ym, 0.0f);
+    }
+    cost[i] = out;
+}
+
+// One wavefront per search: the clear sample of lowest LBFLoss (k_search_cost), ties to the
 // first, replaces the best when strictly lower (search.rs:53-66, 86-92); uniform rounds then
 // tighten the sampling box (LBFLoss::tighten_sample_bbox, loss.rs:33-41).
 __global__ __launch_bounds__(64) void k_search_select(const SearchDesc* __restrict__ d, SearchState* __restrict__ st,
                                                     uint32_t n, uint32_t K, uint32_t round, uint32_t local,
                                                     const float* __restrict__ m6, const uint32_t* __restrict__ ridx,
-                                                    const uint8_t* __restrict__ status, uint8_t* __restrict__ status_all) {
+                                                    const uint8_t* __restrict__ status, const float4* __restrict__ cost,
+                                                    uint8_t* __restrict__ status_all) {
     const uint32_t s = blockIdx.x, lane = threadIdx.x;
     if (s >= n) return;
-    const SearchDesc sd = d[s];
     const SearchState ss0 = st[s];
     const bool active = !local || ss0.found;
     float best = INFINITY, bxm = 0.0f, bym = 0.0f;
     uint32_t bj = 0xffffffffu;
     for (uint32_t j = lane; j < K; j += 64) {
         const uint64_t i = (uint64_t)s * K + j;
-        const uint8_t sv = status[i];
-        status_all[(uint64_t)round * n * K + i] = sv;
-        if (!active || sv != JG_Q_CLEAR) continue;
-        // bbox of the transformed shape (simple_polygon.rs:122-133, 226-253): x_max and y_max
-        const Mat m = load_mat(m6, i);
-        float xm = -3.40282347e+38f, ym = -3.40282347e+38f;
-        for (uint32_t k = 0; k < sd.n_v; k++) {
-            const float2 p = xform(m, sd.v[k].x, sd.v[k].y);
-            xm = mx(xm, p.x);
-            ym = mx(ym, p.y);
-        }
-        const float cost = xm * 10.0f + ym;
-        if (cost < best) { best = cost; bj = j; bxm = xm; bym = ym; }
+        status_all[(uint64_t)round * n * K + i] = status[i];
+        const float4 c = cost[i];
+        if (c.x < best) { best = c.x; bj = j; bxm = c.y; bym = c.z; }
     }
     for (int o = 32; o > 0; o >>= 1) {
         const float oc = __shfl_xor(best, o, 64), ox = __shfl_xor(bxm, o, 64), oy = __shfl_xor(bym, o, 64);
@@ -159,7 +179,8 @@ struct jg_search {
     jg_search_config cfg{};
     jg_plan* plan = nullptr;
     DevBlob blob;
-    size_t o_desc = 0, o_state = 0, o_rot = 0, o_m6 = 0, o_ridx = 0, o_status = 0, o_status_all = 0, o_dt = 0;
+    size_t o_desc = 0, o_state = 0, o_rot = 0, o_m6 = 0, o_ridx = 0, o_status = 0, o_status_all = 0, o_dt = 0,
+           o_cost = 0;
     uint32_t rounds = 0;
     std::vector<uint64_t> generations;
     hipGraph_t graph = nullptr;
@@ -194,7 +215,8 @@ jg_status search_prepare(jg_search* S) {
     S->o_status = S->o_ridx + al(4 * nk);
     S->o_status_all = S->o_status + al(nk);
     S->o_dt = S->o_status_all + al(all);
-    const size_t total = S->o_dt + 12 * all;
+    S->o_cost = S->o_dt + al(12 * all);
+    const size_t total = S->o_cost + 16 * nk;
     if (S->blob.ptr) {
         st = hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
         if (st != JG_OK) return st;
@@ -235,6 +257,7 @@ jg_status search_enqueue(jg_search* S, hipStream_t stream) {
     uint8_t* status = base + S->o_status;
     uint8_t* status_all = base + S->o_status_all;
     float* dt = (float*)(base + S->o_dt);
+    float4* cost = (float4*)(base + S->o_cost);
     hipLaunchKernelGGL(k_search_reset, dim3((n + 63) / 64), dim3(64), 0, stream, desc, state, n);
     const uint64_t nk = (uint64_t)n * K;
     const uint32_t blocks = (uint32_t)((nk + SAMPLE_THREADS - 1) / SAMPLE_THREADS);
@@ -252,8 +275,12 @@ jg_status search_enqueue(jg_search* S, hipStream_t stream) {
                            rot, n, K, r, local ? 1u : 0u, sd_t, sd_r, S->cfg.seed, m6, ridx, dt);
         jg_status st = plan_enqueue(S->plan, m6, status, stream);
         if (st != JG_OK) return st;
+        hipLaunchKernelGGL(k_search_cost, dim3(blocks), dim3(SAMPLE_THREADS), 0, stream, desc,
+                           (const SearchState*)state, n, K, local ? 1u : 0u, (const float*)m6, (const uint8_t*)status,
+                           cost);
         hipLaunchKernelGGL(k_search_select, dim3(n), dim3(64), 0, stream, desc, state, n, K, r, local ? 1u : 0u,
-                           (const float*)m6, (const uint32_t*)ridx, (const uint8_t*)status, status_all);
+                           (const float*)m6, (const uint32_t*)ridx, (const uint8_t*)status, (const float4*)cost,
+                           status_all);
     }
     return hip_check(hipGetLastError(), "search launch");
 }
