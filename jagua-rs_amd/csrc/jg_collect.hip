@@ -50,6 +50,7 @@ constexpr uint32_t CG = WAVE / CP;       // lanes per query in the set-up, conta
 constexpr uint32_t SET_CAP = 16;         // slots per query collector in LDS
 constexpr uint32_t NC = 8;              // candidates per query (bits of an edge's mask)
 constexpr uint32_t CE_CAP = 128;         // edge-test queue entries
+constexpr uint32_t CEG = 4;              // hazard edges per edge-test entry (first edge | (count - 1) << 29)
 constexpr uint32_t CE_LIMIT = 128;       // edge tests one visit pushes: fewer (also below the scan's 16 bits)
 static_assert(64u * (CE_LIMIT - 1u) < 65536u, "a wave's edge pushes must fit 16 bits");
 constexpr bool INLINE_EDGES = false;   // test a visit's hazard edges in the visiting lane (no edge queue)
@@ -61,7 +62,7 @@ constexpr uint32_t C_SLOT = 0x3fffffffu;
 
 struct CollectLds {
     uint32_t nodes[WL_NCAP];             // entity << 24 | node
-    uint32_t edges[CE_CAP];              // hazard edge index
+    uint32_t edges[CE_CAP];              // hazard edges: first edge | (count - 1) << 29 (CEG)
     uint32_t eslot[CE_CAP];              // its hazard's slot
     float4 geom[WL_ENT];                 // circle (cx, cy, r, -) or edge (x1 y1 x2 y2)
     uint32_t emask[WL_ENT];              // query edge: its candidates; pole or pier: SURR_MASK
@@ -201,9 +202,18 @@ __device__ __forceinline__ void wave_collect(const CdeView& c, CollectLds& L, ui
                 const uint32_t at = ne - 1u - lane;
                 const uint32_t o = L.edge_ent[at], sl = L.eslot[at];
                 const uint32_t es = L.ent_slot[o], qs = es & 63u;
-                const float4 he = ldg4(c.edges + L.edges[at]);
-                if (!L.ovf[qs] && !set_has(L, qs, sl) && edge_hit((es & WL_CIRCLE) != 0u, L.geom[o], he))
-                    set_put(L, qs, sl);
+                const uint32_t v = L.edges[at], e0 = v & WL_EDGE_INDEX, k_n = (v >> 29) + 1u;
+                float4 he[CEG];
+#pragma unroll
+                for (uint32_t k = 0; k < CEG; k++) he[k] = ldg4(c.edges + e0 + (k < k_n ? k : 0u));
+                if (!L.ovf[qs] && !set_has(L, qs, sl)) {
+                    const bool circle = (es & WL_CIRCLE) != 0u;
+                    const float4 g = L.geom[o];
+                    bool h = false;
+#pragma unroll
+                    for (uint32_t k = 0; k < CEG; k++) h = h || (k < k_n && edge_hit(circle, g, he[k]));
+                    if (h) set_put(L, qs, sl);
+                }
             }
             ne -= take;
             continue;
@@ -256,7 +266,7 @@ __device__ __forceinline__ void wave_collect(const CdeView& c, CollectLds& L, ui
                         if (e.slot_flags & ENTIRE_BIT) {
                             if (!set_has(L, qs, sl)) set_put(L, qs, sl);
                         } else if (ff_pass(e, circle, g)) {
-                            n_edges += e.edge_count;
+                            n_edges += (e.edge_count + CEG - 1u) / CEG;
                         }
                     }
                     if (n_edges >= CE_LIMIT) {
@@ -285,8 +295,9 @@ __device__ __forceinline__ void wave_collect(const CdeView& c, CollectLds& L, ui
                     const DevEntry e = ldg_ent(c.ent + i);
                     if (!tests_edges(e, circle, g)) continue;
                     const uint32_t sl = e.slot_flags & SLOT_MASK;
-                    for (uint32_t j = 0; j < e.edge_count; j++, at++) {
-                        L.edges[at] = e.edge_begin + j;
+                    for (uint32_t j = 0; j < e.edge_count; j += CEG, at++) {
+                        const uint32_t k = e.edge_count - j < CEG ? e.edge_count - j : CEG;
+                        L.edges[at] = (e.edge_begin + j) | ((k - 1u) << 29);
                         L.eslot[at] = sl;
                         L.edge_ent[at] = (uint8_t)o;
                     }
