@@ -237,6 +237,7 @@ __device__ __forceinline__ void wave_collect(const CdeView& c, CollectLds& L, ui
         }
         const DevNode nd = ldg_node(c.nodes + (live ? (item & WL_NODE_MASK) : 0u));
         uint32_t m = 0, n_edges = 0;
+        uint32_t n_pass = 0, p_beg = 0, p_cnt = 0, p_slot = 0;   // passing Partial entries; the first one's edges
         int fc = 0;
         bool circle = false;
         float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -266,6 +267,12 @@ __device__ __forceinline__ void wave_collect(const CdeView& c, CollectLds& L, ui
                         if (e.slot_flags & ENTIRE_BIT) {
                             if (!set_has(L, qs, sl)) set_put(L, qs, sl);
                         } else if (ff_pass(e, circle, g)) {
+                            if (n_pass == 0u) {
+                                p_beg = e.edge_begin;
+                                p_cnt = e.edge_count;
+                                p_slot = sl;
+                            }
+                            n_pass++;
                             n_edges += (e.edge_count + CEG - 1u) / CEG;
                         }
                     }
@@ -289,7 +296,16 @@ __device__ __forceinline__ void wave_collect(const CdeView& c, CollectLds& L, ui
         const uint32_t e_tot = tot >> 16;
         const bool fits = ne + e_tot <= CE_CAP;
         if (n_edges) {
-            if (fits) {
+            if (fits && n_pass == 1u) {
+                // one passing entry (the common visit): its edges from the count above
+                uint32_t at = ne + (excl >> 16);
+                for (uint32_t j = 0; j < p_cnt; j += CEG, at++) {
+                    const uint32_t k = p_cnt - j < CEG ? p_cnt - j : CEG;
+                    L.edges[at] = (p_beg + j) | ((k - 1u) << 29);
+                    L.eslot[at] = p_slot;
+                    L.edge_ent[at] = (uint8_t)o;
+                }
+            } else if (fits) {
                 uint32_t at = ne + (excl >> 16);
                 for (uint32_t i = nd.haz_begin; i < nd.haz_begin + cnt; i++) {
                     const DevEntry e = ldg_ent(c.ent + i);
