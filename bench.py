@@ -471,12 +471,29 @@ def overlap_c4(stream):
     ms_fwd = timed(lambda: ov.forward(pose, qbuf), 50, stream)
     ms_bwd = timed(lambda: ov.backward(pose, gbuf), 50, stream)
     ms_ov = timed(ov_step, 50, stream)
+    # the evaluation as one graph (forward + backward: seven small launches), replayed
+    side = torch.cuda.Stream()
+    qg, gg = torch.empty_like(qbuf), torch.empty_like(gbuf)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(side):
+        ov.forward(pose, qg, stream=side)
+        ov.backward(pose, gg, stream=side)
+        side.synchronize()
+        with J.Graph.capture(side) as graph:
+            ov.forward(pose, qg, stream=side)
+            ov.backward(pose, gg, stream=side)
+    graph.launch(side)
+    side.synchronize()
+    graph_ok = bool(torch.equal(qg, qbuf) and torch.equal(gg, gbuf))
+    ms_graph = timed(lambda: graph.launch(side), 50, side)
     work = overlap_work(w["poles"], w["off"], pairs, len(w["pose"]), len(w["cedges"]))
     tf_fwd = work["flops_fwd"] / (ms_fwd / 1e3) / 1e12
     tf_bwd = work["flops_bwd"] / (ms_bwd / 1e3) / 1e12
     log(f"overlap: {n_pairs} quadtree pairs (grid {len(w['pairs'])}); fwd {ms_fwd:.3f} ms ({tf_fwd:.2f} TFLOP/s), "
-        f"bwd {ms_bwd:.3f} ms ({tf_bwd:.2f} TFLOP/s), {1e3 / ms_ov:.0f} evals/s")
+        f"bwd {ms_bwd:.3f} ms ({tf_bwd:.2f} TFLOP/s), {1e3 / ms_ov:.0f} evals/s; as one graph {1e3 / ms_graph:.0f} "
+        f"evals/s (equal: {graph_ok})")
     return {"evals_per_s": 1e3 / ms_ov, "ms_per_eval": ms_ov, "fwd_ms": ms_fwd, "bwd_ms": ms_bwd,
+            "graph_evals_per_s": 1e3 / ms_graph, "graph_ms_per_eval": ms_graph, "graph_matches_eager": graph_ok,
             "n_items": int(len(w["pose"])), "n_poles": int(len(w["poles"])), "n_pairs": int(n_pairs),
             "pairs": "engine quadtree, on the GPU", "dtype": "f32",
             "roofline": {"bound": "valu", "unit": "TFLOP/s", "peak": FP32_PEAK_TFLOPS,
