@@ -462,26 +462,29 @@ def overlap_c4(stream):
     pairs = ov.pairs()
     qbuf = torch.empty(1, dtype=torch.float32, device="cuda")
     gbuf = torch.empty((len(w["pose"]), 3), dtype=torch.float32, device="cuda")
+    q1, g1 = torch.empty_like(qbuf), torch.empty_like(gbuf)
 
-    def ov_step():
-        ov.forward(pose, qbuf)
-        ov.backward(pose, gbuf)
+    def ov_step():   # one evaluation: Q and its gradient in one pass over the pairs
+        ov.forward_backward(pose, qbuf, gbuf)
 
     ov_step()
-    ms_fwd = timed(lambda: ov.forward(pose, qbuf), 50, stream)
-    ms_bwd = timed(lambda: ov.backward(pose, gbuf), 50, stream)
+    ov.forward(pose, q1)
+    ov.backward(pose, g1)
+    torch.cuda.synchronize()
+    one_pass_ok = bool(torch.equal(q1, qbuf) and torch.equal(g1, gbuf))
+    ms_fwd = timed(lambda: ov.forward(pose, q1), 50, stream)
+    ms_bwd = timed(lambda: ov.backward(pose, g1), 50, stream)
+    ms_sep = timed(lambda: (ov.forward(pose, q1), ov.backward(pose, g1)), 50, stream)
     ms_ov = timed(ov_step, 50, stream)
-    # the evaluation as one graph (forward + backward: seven small launches), replayed
+    # the evaluation as one graph, replayed
     side = torch.cuda.Stream()
     qg, gg = torch.empty_like(qbuf), torch.empty_like(gbuf)
     torch.cuda.synchronize()
     with torch.cuda.stream(side):
-        ov.forward(pose, qg, stream=side)
-        ov.backward(pose, gg, stream=side)
+        ov.forward_backward(pose, qg, gg, stream=side)
         side.synchronize()
         with J.Graph.capture(side) as graph:
-            ov.forward(pose, qg, stream=side)
-            ov.backward(pose, gg, stream=side)
+            ov.forward_backward(pose, qg, gg, stream=side)
     graph.launch(side)
     side.synchronize()
     graph_ok = bool(torch.equal(qg, qbuf) and torch.equal(gg, gbuf))
@@ -489,18 +492,24 @@ def overlap_c4(stream):
     work = overlap_work(w["poles"], w["off"], pairs, len(w["pose"]), len(w["cedges"]))
     tf_fwd = work["flops_fwd"] / (ms_fwd / 1e3) / 1e12
     tf_bwd = work["flops_bwd"] / (ms_bwd / 1e3) / 1e12
-    log(f"overlap: {n_pairs} quadtree pairs (grid {len(w['pairs'])}); fwd {ms_fwd:.3f} ms ({tf_fwd:.2f} TFLOP/s), "
-        f"bwd {ms_bwd:.3f} ms ({tf_bwd:.2f} TFLOP/s), {1e3 / ms_ov:.0f} evals/s; as one graph {1e3 / ms_graph:.0f} "
-        f"evals/s (equal: {graph_ok})")
-    return {"evals_per_s": 1e3 / ms_ov, "ms_per_eval": ms_ov, "fwd_ms": ms_fwd, "bwd_ms": ms_bwd,
+    tf_eval = work["flops_bwd"] / (ms_ov / 1e3) / 1e12
+    log(f"overlap: {n_pairs} quadtree pairs (grid {len(w['pairs'])}); one pass {ms_ov * 1e3:.1f} us "
+        f"({1e3 / ms_ov:.0f} evals/s, {tf_eval:.2f} TFLOP/s, equal to the separate passes: {one_pass_ok}); "
+        f"fwd {ms_fwd:.4f} ms ({tf_fwd:.2f} TFLOP/s), bwd {ms_bwd:.4f} ms ({tf_bwd:.2f} TFLOP/s); "
+        f"as one graph {1e3 / ms_graph:.0f} evals/s (equal: {graph_ok})")
+    return {"evals_per_s": 1e3 / ms_ov, "ms_per_eval": ms_ov, "form": "forward and backward in one pass "
+            "(jg_overlap_fwd_bwd: two launches)", "one_pass_matches_separate": one_pass_ok,
+            "fwd_ms": ms_fwd, "bwd_ms": ms_bwd, "separate_ms_per_eval": ms_sep,
             "graph_evals_per_s": 1e3 / ms_graph, "graph_ms_per_eval": ms_graph, "graph_matches_eager": graph_ok,
             "n_items": int(len(w["pose"])), "n_poles": int(len(w["poles"])), "n_pairs": int(n_pairs),
             "pairs": "engine quadtree, on the GPU", "dtype": "f32",
             "roofline": {"bound": "valu", "unit": "TFLOP/s", "peak": FP32_PEAK_TFLOPS,
+                         "achieved_eval": tf_eval, "frac_eval": tf_eval / FP32_PEAK_TFLOPS,
+                         "flops_eval": work["flops_bwd"],
                          "achieved_fwd": tf_fwd, "achieved_bwd": tf_bwd, "frac_fwd": tf_fwd / FP32_PEAK_TFLOPS,
                          "frac_bwd": tf_bwd / FP32_PEAK_TFLOPS,
                          "gbs_fwd": work["bytes_fwd"] / (ms_fwd / 1e3) / 1e9,
-                         "gbs_bwd": work["bytes_bwd"] / (ms_bwd / 1e3) / 1e9, **work}}
+            "gbs_bwd": work["bytes_bwd"] / (ms_bwd / 1e3) / 1e9, **work}}
 
 
 # ---------------------------------------------------------------------------- config 3
@@ -664,9 +673,8 @@ def run_c5(args, summary=False):
     t_pairs = time.time() - t0
     qb = torch.empty(1, dtype=torch.float32, device="cuda")
     gb = torch.empty((len(pose), 3), dtype=torch.float32, device="cuda")
-    ov.forward(pose_t, qb)
-    ov.backward(pose_t, gb)
-    ms_grad = timed(lambda: (ov.forward(pose_t, qb), ov.backward(pose_t, gb)), 20, stream)
+    ov.forward_backward(pose_t, qb, gb)
+    ms_grad = timed(lambda: ov.forward_backward(pose_t, qb, gb), 20, stream)
     log(f"c5 gradient pass: {ms_grad:.3f} ms ({n_pairs} candidate pairs from the quadtree in {t_pairs:.2f} s), "
         f"Q = {qb.item():.4g}")
     # the three in one graph: captured on a side stream after one eager call each
@@ -678,14 +686,12 @@ def run_c5(args, summary=False):
     with torch.cuda.stream(side):
         plan.detect(mats, status_g, stream=side)
         plan.collect(mats, capacity=cap, count=cnt_g, hash=hsh_g, keys=keys_g, stream=side)
-        ov.forward(pose_t, qg, stream=side)
-        ov.backward(pose_t, gg, stream=side)
+        ov.forward_backward(pose_t, qg, gg, stream=side)
         side.synchronize()
         with J.Graph.capture(side) as graph:
             plan.detect(mats, status_g, stream=side)
             plan.collect(mats, capacity=cap, count=cnt_g, hash=hsh_g, keys=keys_g, stream=side)
-            ov.forward(pose_t, qg, stream=side)
-            ov.backward(pose_t, gg, stream=side)
+            ov.forward_backward(pose_t, qg, gg, stream=side)
     graph_ok = True
     for _ in range(2):
         status_g.fill_(255)

@@ -404,6 +404,10 @@ jg_status jg_overlap_set_pairs(jg_overlap* ov, const uint32_t* pairs, uint64_t n
 jg_status jg_overlap_fwd(jg_overlap* ov, const float* d_pose, float* d_q, void* stream);
 /* Backward: dQ/d(x, y, theta) per item into d_grad device [n_items][3]. */
 jg_status jg_overlap_bwd(jg_overlap* ov, const float* d_pose, float* d_grad, void* stream);
+/* Forward and backward in one pass over the pairs: Q into d_q and the gradient into d_grad,
+ * bit for bit those of jg_overlap_fwd and jg_overlap_bwd (one evaluation of an optimiser
+ * step). */
+jg_status jg_overlap_fwd_bwd(jg_overlap* ov, const float* d_pose, float* d_q, float* d_grad, void* stream);
 void jg_overlap_free(jg_overlap* ov);
 /* Candidate pairs from the engine's quadtree, found on the GPU, replacing the overlap's pairs:
  * item k is the hazard of slot item_slots[k] in cde, with bounding circle centre (x, y) of its

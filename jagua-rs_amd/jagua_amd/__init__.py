@@ -708,6 +708,17 @@ class PoleOverlap:
                                      _stream_ptr(stream)))
         return out
 
+    def forward_backward(self, pose, q=None, grad=None, stream=None):
+        """(Q, dQ/d(x, y, theta)) in one pass, equal to forward() and backward()."""
+        import torch
+        if q is None:
+            q = torch.empty(1, dtype=torch.float32, device=pose.device)
+        if grad is None:
+            grad = torch.empty((self.n_items, 3), dtype=torch.float32, device=pose.device)
+        check(N.lib().jg_overlap_fwd_bwd(self._h, C.c_void_p(pose.data_ptr()), C.c_void_p(q.data_ptr()),
+                                         C.c_void_p(grad.data_ptr()), _stream_ptr(stream)))
+        return q, grad
+
 
 def candidate_pairs(pose, radius, bbox, depth=8):
     """Items whose bounding circles overlap (uniform grid of 2^depth cells per side)."""

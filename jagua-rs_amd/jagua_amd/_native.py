@@ -129,6 +129,7 @@ _SIGS = {
     "jg_overlap_set_pairs": (C.c_int32, [c_vp, c_u32p, C.c_uint64]),
     "jg_overlap_fwd": (C.c_int32, [c_vp, c_vp, c_vp, c_vp]),
     "jg_overlap_bwd": (C.c_int32, [c_vp, c_vp, c_vp, c_vp]),
+    "jg_overlap_fwd_bwd": (C.c_int32, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     "jg_overlap_free": (None, [c_vp]),
     "jg_overlap_pairs_from_cde": (C.c_int32, [c_vp, c_vp, c_u32p, c_vp, c_f32p, c_u64p, c_vp]),
     "jg_overlap_get_pairs": (C.c_int32, [c_vp, c_u32p, C.c_uint64, c_u64p]),

@@ -41,31 +41,34 @@ def test_detect_collect_overlap_in_one_graph(cuda):
                     torch.empty(n, dtype=torch.int64, device=cuda))
     keys_out = torch.empty((n, 6), dtype=torch.int32, device=cuda)
     q, g = torch.empty(1, dtype=torch.float32, device=cuda), torch.empty((len(pose), 3), dtype=torch.float32, device=cuda)
+    q1, g1 = torch.empty_like(q), torch.empty_like(g)
 
     def calls():
         plan.detect(m, st, stream=side)
         plan.collect(m, capacity=6, count=cnt, hash=hsh, keys=keys_out, stream=side)
         ov.forward(pose, q, stream=side)
         ov.backward(pose, g, stream=side)
+        ov.forward_backward(pose, q1, g1, stream=side)
 
     torch.cuda.synchronize()
     with torch.cuda.stream(side):
         calls()                       # builds every table and buffer before the capture
         side.synchronize()
-        eager = [t.clone() for t in (st, cnt, hsh, keys_out, q, g)]
+        eager = [t.clone() for t in (st, cnt, hsh, keys_out, q, g, q1, g1)]
         with J.Graph.capture(side) as graph:
             calls()
     np.testing.assert_array_equal(eager[0].cpu().numpy(), np.concatenate([e["status"] for e in expect]))
     np.testing.assert_array_equal(eager[1].cpu().numpy().astype(np.uint32), np.concatenate([e["count"] for e in expect]))
     np.testing.assert_array_equal(eager[2].cpu().numpy().view(np.uint64), np.concatenate([e["hash"] for e in expect]))
     np.testing.assert_array_equal(eager[3].cpu().numpy().view(np.uint32), np.concatenate([e["keys"] for e in expect]))
+    assert torch.equal(eager[6], eager[4]) and torch.equal(eager[7], eager[5])   # one pass = two passes
     for r in range(4):
-        for t in (st, cnt, hsh, keys_out, q, g):
+        for t in (st, cnt, hsh, keys_out, q, g, q1, g1):
             t.fill_(0 if t.dtype == torch.uint8 else -1)
         x = torch.randn(1000, device=cuda).sum()      # eager work between replays
         torch.cuda.synchronize()
         graph.launch(side)
         side.synchronize()
-        for t, e in zip((st, cnt, hsh, keys_out, q, g), eager):
+        for t, e in zip((st, cnt, hsh, keys_out, q, g, q1, g1), eager):
             assert torch.equal(t, e), f"replay {r} differs"
         del x

@@ -43,6 +43,28 @@ def test_overlap_vs_fp64(cuda, n_items):
     # determinism
     assert ov.forward(pose).item() == q
     assert np.array_equal(ov.backward(pose).cpu().numpy().astype(np.float64), g)
+    # the one-pass form equals the two passes bit for bit
+    q2, g2 = ov.forward_backward(pose)
+    assert q2.item() == q and np.array_equal(g2.cpu().numpy().astype(np.float64), g)
+
+
+def test_overlap_items_of_many_poles(cuda):
+    """Items of 60-150 poles: pairs whose pole sets span several 64-pole chunks on both sides."""
+    w = S.config4_workload(seed=9, n_items=96, width=160.0, height=80.0, poles_lo=60, poles_hi=150)
+    n = np.diff(w["off"].astype(np.int64))
+    assert n.max() > 128 and len(w["pairs"]) > 0
+    ov = J.PoleOverlap(w["poles"], w["off"], w["pairs"], w["cedges"])
+    pose = torch.from_numpy(w["pose"]).to(cuda)
+    q, g = ov.forward_backward(pose)
+    q, g = q.item(), g.cpu().numpy().astype(np.float64)
+    q64, g64 = O.overlap64(w["pose"].astype(np.float64), w["poles"], w["off"], w["pairs"], w["cedges"])
+    glob, scaled, comp = errors(g, g64)
+    print(f"poles {n.min()}-{n.max()}, pairs {len(w['pairs'])}: Q rel err {abs(q - q64) / q64:.2e}, grad L2 "
+          f"{glob:.2e}, max/scale {scaled:.2e}")
+    assert abs(q - q64) / q64 < 1e-5, (q, q64)
+    assert glob < 1e-4 and scaled < 1e-4
+    assert ov.forward(pose).item() == q
+    assert np.array_equal(ov.backward(pose).cpu().numpy().astype(np.float64), g)
 
 
 @pytest.mark.parametrize("h,tol", [(1e-3, 2e-3), (1e-5, 1e-4)])
