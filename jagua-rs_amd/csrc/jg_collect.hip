@@ -73,6 +73,11 @@ struct CollectLds {
     uint32_t cdone[CP];                  // bit i: candidate i is in the set
     uint32_t ovf[CP];                    // per query: its set outgrew SET_CAP
     int32_t vroot[CP];
+    uint32_t vdepth[CP];                 // its depth
+    uint32_t vcoll[CP];                  // whether it collects (a leaf, or at most cd_threshold active edges)
+    uint32_t vcell[CP];                  // its cell at its level, iy << 16 | ix (levels up to the tables')
+    float4 vbox[CP];                     // its box
+    float mat[CP][6];                    // per query: its transformation
     float4 qbox[CP];                     // per query: its transformed bbox
     uint32_t vbeg[CP], vcnt[CP];         // its virtual root's entry range
     uint32_t vpre[CP + 1];               // prefix sums of vcnt over the pass
@@ -137,6 +142,55 @@ __device__ __forceinline__ bool box_inside(float4 in, float4 out) {
 __device__ __forceinline__ bool may_collect(const CdeView& c, uint32_t slot, float4 eb) {
     if (slot == c.exterior_slot) return !box_inside(eb, inner_shrunk(c));
     return boxes_meet(eb, widened(ldg4(c.haz_bbox + slot)));
+}
+
+// The node from which a query edge with box eb, starting at node n, reaches the same visits
+// of QTNode::collect_collisions (qt_node.rs:151-191): while n is an internal node of the
+// collection (it has children and more than cd_threshold active edges) and eb lies strictly
+// inside one of its quadrants, that quadrant's child.  An edge strictly inside a quadrant
+// collides with no other quadrant (the box test of collides_with_quadrants,
+// qt_traits.rs:19-106), so the visit of n would push just that child.
+__device__ __forceinline__ uint32_t collect_start(const CdeView& c, uint32_t n, float4 eb) {
+    for (;;) {
+        const DevNode nd = ldg_node(c.nodes + n);
+        if (nd.first_child < 0 || nd.n_active_edges <= c.cd_threshold) return n;
+        const float4 r = nd.bbox;
+        const float mxm = (r.x + r.z) / 2.0f, mym = (r.y + r.w) / 2.0f;
+        const bool xh = eb.x > mxm, xl = eb.z < mxm, yh = eb.y > mym, yl = eb.w < mym;
+        if (!(xh || xl) || !(yh || yl)) return n;
+        // children in quadrant order: top right, top left, bottom left, bottom right
+        n = (uint32_t)nd.first_child + (xh ? (yh ? 0u : 3u) : (yh ? 1u : 2u));
+    }
+}
+
+// collect_start for a query edge with box eb, of a query whose virtual root vr (depth vd, box
+// vbox, cell (vx, vy) of its level) does not collect, through the cell tables: below vr the
+// edge's root path is followed arithmetically while eb lies strictly inside one quadrant (no
+// memory read), down to the tables' deepest level; that cell's entry gives the deepest node on
+// the path and the depth of the first collecting node on it (below vr, as vr does not
+// collect), and at most one more entry gives that node.  Without one, the start is the
+// deepest node, from which collect_start goes on below the tables' levels.  The path's boxes
+// below vr hold the edge: vr's box does, and eb is strictly on one side of each midline.
+__device__ __forceinline__ uint32_t collect_start_cells(const CdeView& c, float4 eb, uint32_t vr, uint32_t vd,
+                                                        float4 vbox, uint32_t vx, uint32_t vy) {
+    if (vd >= c.cell_level) return collect_start(c, vr, eb);
+    float4 r = vbox;
+    uint32_t ix = vx, iy = vy, l = vd;
+    for (; l < c.cell_level; l++) {
+        const float mxm = (r.x + r.z) / 2.0f, mym = (r.y + r.w) / 2.0f;
+        const bool xh = eb.x > mxm, xl = eb.z < mxm, yh = eb.y > mym, yl = eb.w < mym;
+        if (!(xh || xl) || !(yh || yl)) break;
+        r = make_float4(xh ? mxm : r.x, yh ? mym : r.y, xh ? r.z : mxm, yh ? r.w : mym);
+        ix = 2u * ix + (xh ? 1u : 0u);
+        iy = 2u * iy + (yh ? 1u : 0u);
+    }
+    if (l == vd) return vr;                       // eb meets a midline of vr: vr's visit starts it
+    const uint32_t e = ldg1(c.cells + ((1u << (2u * l)) - 1u) / 3u + ((iy << l) | ix));
+    const uint32_t cd = cell_cdepth(e), n = e & CELL_NODE;
+    if (cd == 15u) return l == c.cell_level ? collect_start(c, n, eb) : n;
+    if (cd == cell_depth(e)) return n;
+    const uint32_t s = l - cd;                    // the first collecting node, above the cell's node
+    return ldg1(c.cells + ((1u << (2u * cd)) - 1u) / 3u + (((iy >> s) << cd) | (ix >> s))) & CELL_NODE;
 }
 
 __device__ __forceinline__ float4 entity_box(bool circle, float4 g) {
@@ -392,15 +446,37 @@ __device__ __forceinline__ void collect_pass(const SetDesc& sd, uint32_t set_ind
         L.cdone[qs] = 0u;
     }
     int vr = 0;
-    uint32_t vcnt = 0, ve0 = 0;
+    uint32_t vcnt = 0, ve0 = 0, vdepth = 0, vcell = 0, vcoll = 1;
+    float4 vbox = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 root_bb = ldg4((const float4*)c.nodes);
     if (active) {
-        vr = virtual_root(c, bb);
+        vr = virtual_root_cells(c, root_bb, bb, vdepth);
         const DevNode vn = ldg_node(c.nodes + vr);
         ve0 = vn.haz_begin;
         vcnt = vn.haz_info & COUNT_MASK;
+        vbox = vn.bbox;
+        vcoll = (vn.first_child < 0 || vn.n_active_edges <= c.cd_threshold) ? 1u : 0u;
+        // vr's cell: the quadrants of the root box holding vr's centre (vr's box is one of them)
+        const float cx = (vbox.x + vbox.z) / 2.0f, cy = (vbox.y + vbox.w) / 2.0f;
+        float4 r = root_bb;
+        uint32_t ix = 0, iy = 0;
+        for (uint32_t l = 0; l < vdepth && l < c.cell_level; l++) {
+            const float mxm = (r.x + r.z) / 2.0f, mym = (r.y + r.w) / 2.0f;
+            const bool xh = cx > mxm, yh = cy > mym;
+            r = make_float4(xh ? mxm : r.x, yh ? mym : r.y, xh ? r.z : mxm, yh ? r.w : mym);
+            ix = 2u * ix + (xh ? 1u : 0u);
+            iy = 2u * iy + (yh ? 1u : 0u);
+        }
+        vcell = (iy << 16) | ix;
     }
     if (sub == 0) {
         L.vroot[qs] = vr;
+        L.vdepth[qs] = vdepth;
+        L.vcoll[qs] = vcoll;
+        L.vcell[qs] = vcell;
+        L.vbox[qs] = vbox;
+        L.mat[qs][0] = m.a; L.mat[qs][1] = m.b; L.mat[qs][2] = m.c;
+        L.mat[qs][3] = m.d; L.mat[qs][4] = m.e; L.mat[qs][5] = m.f;
         L.qbox[qs] = bb;
         L.vbeg[qs] = ve0;
         L.vcnt[qs] = vcnt;
@@ -463,6 +539,7 @@ __device__ __forceinline__ void collect_pass(const SetDesc& sd, uint32_t set_ind
     const uint32_t n_ff = it.n_poles + it.n_piers;
     const uint32_t n_ent = n_ff + it.n_v;
     const uint32_t total = P * n_ent;
+    const float inv_ent = 1.0f / (float)n_ent;
     uint32_t cursor = 0;
     while (cursor < total) {
         wave_sync();
@@ -474,10 +551,13 @@ __device__ __forceinline__ void collect_pass(const SetDesc& sd, uint32_t set_ind
             bool circle = false;
             float4 gm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (e < total) {
-                qi = e / n_ent;
+                // e / n_ent through the float reciprocal: the quotient is below CP, so the
+                // estimate is off by at most one, corrected here
+                qi = (uint32_t)((float)e * inv_ent);
+                qi = qi * n_ent > e ? qi - 1u : ((qi + 1u) * n_ent <= e ? qi + 1u : qi);
                 const uint32_t k = e - qi * n_ent;
                 if (!L.ovf[qi]) {
-                    const Mat mq = load_mat(m6, sd.first + q0 + qi);
+                    const Mat mq{L.mat[qi][0], L.mat[qi][1], L.mat[qi][2], L.mat[qi][3], L.mat[qi][4], L.mat[qi][5]};
                     if (k < n_ff) {
                         const float4 p = ldg4(it.ff + k);
                         circle = k < it.n_poles;
@@ -512,6 +592,9 @@ __device__ __forceinline__ void collect_pass(const SetDesc& sd, uint32_t set_ind
                             }
                         }
                         want = (em & ~L.cdone[qi]) != 0u;
+                        if (want && !L.vcoll[qi])
+                            root = collect_start_cells(c, entity_box(false, gm), root, L.vdepth[qi], L.vbox[qi],
+                                                       L.vcell[qi] & 0xffffu, L.vcell[qi] >> 16);
                     }
                 }
             }

@@ -97,7 +97,12 @@ struct CdeView {
                                       // else empty (x_min > x_max)
 };
 
-constexpr uint32_t CELL_NODE = 0xffffffu;    // a cell table entry: depth << 24 | node
+// A cell table entry: cdepth << 28 | depth << 24 | node, for the deepest node on the path to the
+// cell and the depth of the first node on that path where QTNode::collect_collisions tests
+// entries (15: none on the path).
+constexpr uint32_t CELL_NODE = 0xffffffu;
+__device__ __forceinline__ uint32_t cell_depth(uint32_t e) { return (e >> 24) & 15u; }
+__device__ __forceinline__ uint32_t cell_cdepth(uint32_t e) { return e >> 28; }
 
 struct ItemView {
     const float2* v;                  // local vertices (CCW)
@@ -218,7 +223,7 @@ __device__ __forceinline__ int virtual_root_cells(const CdeView& c, float4 root,
         }
         const uint32_t e = ldg1(c.cells + ((1u << (2u * l)) - 1u) / 3u + ((iy << l) | ix));
         n = (int)(e & CELL_NODE);
-        depth = e >> 24;
+        depth = cell_depth(e);
         if (depth < l || l < c.cell_level) return n;
     }
     for (;;) {
@@ -262,7 +267,7 @@ __device__ __forceinline__ uint32_t start_node(const CdeView& c, float4 root, fl
     }
     if (l == 0u) return 0u;
     const uint32_t e = ldg1(c.cells + ((1u << (2u * l)) - 1u) / 3u + ((iy << l) | ix));
-    depth = e >> 24;
+    depth = cell_depth(e);
     return e & CELL_NODE;
 }
 

@@ -719,12 +719,20 @@ static void image_write_slot(PoolImage& im, const SlotMap& map, uint32_t s) {
 
 static uint64_t with_headroom(uint64_t n) { return n + n / 2 + 1024; }
 
+bool collects_at(const QuadTree& qt, int32_t n) {
+    const QNode& q = qt.pool[(size_t)n];
+    return q.child < 0 || q.n_active_edges <= qt.cd_threshold;
+}
+
 namespace {
-// Writes depth << 24 | n for node n (at depth `depth`, covering cell (ix, iy) of the 2^level
-// grid, side cells per side at that level) into the tables of levels level .. cell_level.
-void fill_cells(PoolImage& im, const QuadTree& qt, int32_t n, uint32_t depth, uint32_t level, uint32_t ix, uint32_t iy) {
+// Writes cdepth << 28 | depth << 24 | n for node n (at depth `depth`, covering cell (ix, iy) of
+// the 2^level grid, side cells per side at that level; cdepth the first collecting depth on its
+// path so far) into the tables of levels level .. cell_level.
+void fill_cells(PoolImage& im, const QuadTree& qt, int32_t n, uint32_t depth, uint32_t level, uint32_t ix, uint32_t iy,
+                uint32_t cdepth) {
     const int32_t ch = qt.pool[(size_t)n].child;
-    const uint32_t entry = (depth << 24) | (uint32_t)n;
+    if (cdepth == CELL_NO_CDEPTH && collects_at(qt, n)) cdepth = depth;
+    const uint32_t entry = (cdepth << 28) | (depth << 24) | (uint32_t)n;
     for (uint32_t l = level, s = 1; l <= im.cell_level; l++, s *= 2) {
         // the block of cells of level l under this level-`level` cell
         const size_t off = (((size_t)1 << (2 * l)) - 1) / 3;
@@ -733,10 +741,10 @@ void fill_cells(PoolImage& im, const QuadTree& qt, int32_t n, uint32_t depth, ui
         if (ch >= 0 && l == level && level < im.cell_level) break;     // the children write deeper levels
     }
     if (ch < 0 || level == im.cell_level) return;
-    fill_cells(im, qt, ch + 0, depth + 1, level + 1, 2 * ix + 1, 2 * iy + 1);
-    fill_cells(im, qt, ch + 1, depth + 1, level + 1, 2 * ix, 2 * iy + 1);
-    fill_cells(im, qt, ch + 2, depth + 1, level + 1, 2 * ix, 2 * iy);
-    fill_cells(im, qt, ch + 3, depth + 1, level + 1, 2 * ix + 1, 2 * iy);
+    fill_cells(im, qt, ch + 0, depth + 1, level + 1, 2 * ix + 1, 2 * iy + 1, cdepth);
+    fill_cells(im, qt, ch + 1, depth + 1, level + 1, 2 * ix, 2 * iy + 1, cdepth);
+    fill_cells(im, qt, ch + 2, depth + 1, level + 1, 2 * ix, 2 * iy, cdepth);
+    fill_cells(im, qt, ch + 3, depth + 1, level + 1, 2 * ix + 1, 2 * iy, cdepth);
 }
 }  // namespace
 
@@ -746,7 +754,7 @@ void image_cells(PoolImage& im, const QuadTree& qt) {
     const bool flat = !(r.z > r.x && r.w > r.y);
     im.cell_level = flat ? 0u : std::min<uint32_t>(qt.pool[0].level, CELL_LEVELS_MAX);
     im.cells.assign((((size_t)1 << (2 * (im.cell_level + 1))) - 1) / 3, 0u);
-    fill_cells(im, qt, 0, 0, 0, 0, 0);
+    fill_cells(im, qt, 0, 0, 0, 0, 0, CELL_NO_CDEPTH);
 }
 
 void image_build(PoolImage& im, QuadTree& qt, const SlotMap& map) {
@@ -819,7 +827,14 @@ bool image_patch(PoolImage& im, QuadTree& qt, const SlotMap& map, std::vector<ui
                 nodes.push_back((uint32_t)t);
             }
     }
-    for (uint32_t i : nodes) image_write_node(im, qt, map, i);
+    for (uint32_t i : nodes) {
+        // a node within the tables' levels that starts or stops collecting changes them
+        const FlatNode& old = im.nodes[i];
+        const bool was = old.first_child < 0 || old.n_active_edges <= qt.cd_threshold;
+        if ((uint32_t)(qt.pool[0].level - qt.pool[i].level) <= CELL_LEVELS_MAX && was != collects_at(qt, (int32_t)i))
+            qt.near_root_changed = true;
+        image_write_node(im, qt, map, i);
+    }
     const uint32_t S = (uint32_t)map.slots.size();
     if (im.slot_ver.size() < S) {
         im.hbb.resize(S, make_float4(0, 0, 0, 0));

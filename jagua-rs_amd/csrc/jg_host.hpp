@@ -89,6 +89,7 @@ struct QEntry {                          // QTHazard in one node (qt_hazard.rs:1
 
 // Levels of the cell tables: the tree's depth, at most this many (levels 0 .. 7: 87 KB).
 constexpr uint32_t CELL_LEVELS_MAX = 7;
+constexpr uint32_t CELL_NO_CDEPTH = 15;
 
 struct QNode {                           // QTNode (qt_node.rs:14-25)
     uint8_t level = 0;
@@ -170,8 +171,11 @@ struct PoolImage {
     std::vector<float2> verts;
     std::vector<uint32_t> slot_ver;       // version written per slot, 0 = none
     // cell tables of levels 0 .. cell_level, one after another (level l at (4^l - 1) / 3):
-    // entry (ix, iy) of level l (at iy << l | ix) is depth << 24 | node for the deepest node on
-    // the tree path to cell (ix, iy) of the 2^l grid of the root box's quadrant subdivision
+    // entry (ix, iy) of level l (at iy << l | ix) is cdepth << 28 | depth << 24 | node for the
+    // deepest node on the tree path to cell (ix, iy) of the 2^l grid of the root box's
+    // quadrant subdivision, cdepth the depth of the first node on that path where
+    // QTNode::collect_collisions tests entries (a leaf, or at most cd_threshold active edges;
+    // their descendants are such nodes too), CELL_NO_CDEPTH when there is none
     std::vector<uint32_t> cells;
     uint32_t cell_level = 0;
     uint64_t garbage = 0;                 // entry + edge records no node refers to
@@ -184,6 +188,8 @@ struct PoolImage {
 // Fills im.cells from the tree (quadrant k of a node: 0 = x high y high, 1 = x low y high,
 // 2 = x low y low, 3 = x high y low, as box_quadrant orders them).
 void image_cells(PoolImage& im, const QuadTree& qt);
+// Whether QTNode::collect_collisions tests node n's entries (qt_node.rs:151-191).
+bool collects_at(const QuadTree& qt, int32_t n);
 
 // Full build of the host mirror and the device layout (capacities with headroom).
 void image_build(PoolImage& im, QuadTree& qt, const SlotMap& map);

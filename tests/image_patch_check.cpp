@@ -117,8 +117,9 @@ static void verify(const PoolImage& im, const std::vector<uint8_t>& dev, const Q
         }
     }
     CHECK(i == f.nodes.size(), "reachable nodes %zu vs %zu", i, f.nodes.size());
-    // the cell tables: entry (ix, iy) of level l is depth << 24 | node for the deepest node on the
-    // tree path to that cell
+    // the cell tables: entry (ix, iy) of level l is cdepth << 28 | depth << 24 | node for the
+    // deepest node on the tree path to that cell, cdepth the depth of the path's first node where
+    // collect_collisions tests entries (a leaf or at most cd_threshold active edges), 15 if none
     const uint32_t L = im.cell_level;
     CHECK(L == (qt.pool[0].level < CELL_LEVELS_MAX ? qt.pool[0].level : CELL_LEVELS_MAX), "cell level %u", L);
     for (uint32_t l = 0; l <= L; l++) {
@@ -127,11 +128,17 @@ static void verify(const PoolImage& im, const std::vector<uint8_t>& dev, const Q
             for (uint32_t ix = 0; ix < (1u << l); ix++) {
                 int32_t n = 0;
                 uint32_t d = 0;
+                const auto collects = [&](int32_t k) {
+                    return qt.pool[(size_t)k].child < 0 || qt.pool[(size_t)k].n_active_edges <= qt.cd_threshold;
+                };
+                uint32_t cd = collects(0) ? 0u : CELL_NO_CDEPTH;
                 for (; d < l && qt.pool[(size_t)n].child >= 0; d++) {
                     const uint32_t bx = (ix >> (l - 1u - d)) & 1u, by = (iy >> (l - 1u - d)) & 1u;
                     n = qt.pool[(size_t)n].child + (bx ? (by ? 0 : 3) : (by ? 1 : 2));
+                    if (cd == CELL_NO_CDEPTH && collects(n)) cd = d + 1u;
                 }
-                CHECK(rd<uint32_t>(dev, im.o_ce + 4ull * (off + (((uint64_t)iy << l) | ix))) == ((d << 24) | (uint32_t)n),
+                CHECK(rd<uint32_t>(dev, im.o_ce + 4ull * (off + (((uint64_t)iy << l) | ix))) ==
+                          ((cd << 28) | (d << 24) | (uint32_t)n),
                       "level %u cell (%u, %u)", l, ix, iy);
             }
     }
