@@ -45,6 +45,10 @@ using namespace jg;
 
 namespace {
 
+constexpr uint32_t CL_WAVES = 6;         // waves per SIMD: VGPRs (80) and LDS per wave are sized for it
+constexpr uint32_t CL_ENT = 64;          // entities per traversal (below 256: 8-bit indices)
+constexpr uint32_t CL_NCAP = 320;        // node-visit queue entries
+static_assert(CL_NCAP >= CL_ENT + WL_RESERVE + 3u, "node queue must hold the roots and the reserve");
 constexpr uint32_t CP = 8;              // queries per collection pass
 constexpr uint32_t CG = WAVE / CP;       // lanes per query in the set-up, containment and result phases
 constexpr uint32_t SET_CAP = 16;         // slots per query collector in LDS
@@ -59,13 +63,14 @@ constexpr uint32_t SURR_MASK = 0u;       // emask of a pole or pier: the node-en
 constexpr uint32_t C_EXT = 0x80000000u;  // candidate record flags: exterior hazard, Entire in the virtual root
 constexpr uint32_t C_ENTIRE = 0x40000000u;
 constexpr uint32_t C_SLOT = 0x3fffffffu;
+constexpr uint32_t VR_COLLECTS = 0x80000000u;   // vcell flag: the virtual root collects
 
 struct CollectLds {
-    uint32_t nodes[WL_NCAP];             // entity << 24 | node
+    uint32_t nodes[CL_NCAP];             // entity << 24 | node
     uint32_t edges[CE_CAP];              // hazard edges: first edge | (count - 1) << 29 (CEG)
     uint32_t eslot[CE_CAP];              // its hazard's slot
-    float4 geom[WL_ENT];                 // circle (cx, cy, r, -) or edge (x1 y1 x2 y2)
-    uint32_t emask[WL_ENT];              // query edge: its candidates; pole or pier: SURR_MASK
+    float4 geom[CL_ENT];                 // circle (cx, cy, r, -) or edge (x1 y1 x2 y2)
+    uint32_t emask[CL_ENT];              // query edge: its candidates; pole or pier: SURR_MASK
     uint32_t set[CP * SET_CAP];          // per query: slot + 1, filled from the front, 0 = free
     uint32_t cand[CP * NC];              // per query: candidate slot | C_EXT | C_ENTIRE
     float4 cbox[CP * NC];                // its widened box (the exterior: the inner box, shrunk)
@@ -74,16 +79,17 @@ struct CollectLds {
     uint32_t ovf[CP];                    // per query: its set outgrew SET_CAP
     int32_t vroot[CP];
     uint32_t vdepth[CP];                 // its depth
-    uint32_t vcoll[CP];                  // whether it collects (a leaf, or at most cd_threshold active edges)
-    uint32_t vcell[CP];                  // its cell at its level, iy << 16 | ix (levels up to the tables')
+    uint32_t vcell[CP];                  // its cell at its level, iy << 16 | ix (levels up to the tables'),
+                                         // | VR_COLLECTS when it collects (a leaf, or at most cd_threshold active edges)
     float4 vbox[CP];                     // its box
     float mat[CP][6];                    // per query: its transformation
     float4 qbox[CP];                     // per query: its transformed bbox
     uint32_t vbeg[CP], vcnt[CP];         // its virtual root's entry range
     uint32_t vpre[CP + 1];               // prefix sums of vcnt over the pass
     uint8_t edge_ent[CE_CAP];
-    uint8_t ent_slot[WL_ENT];            // query (0..CP-1) | WL_CIRCLE
+    uint8_t ent_slot[CL_ENT];            // query (0..CP-1) | WL_CIRCLE
 };
+static_assert(sizeof(CollectLds) * 4u * CL_WAVES <= 160u * 1024u, "a CU's waves share its 160 KB of LDS");
 
 __device__ __forceinline__ bool set_has(const CollectLds& L, uint32_t q, uint32_t slot) {
     const uint32_t key = slot + 1u;
@@ -273,8 +279,8 @@ __device__ __forceinline__ void wave_collect(const CdeView& c, CollectLds& L, ui
             continue;
         }
         uint32_t take = 1u;
-        if (nn + 3u + WL_RESERVE <= WL_NCAP) {
-            const uint32_t room = (WL_NCAP - WL_RESERVE - nn) / 3u;
+        if (nn + 3u + WL_RESERVE <= CL_NCAP) {
+            const uint32_t room = (CL_NCAP - WL_RESERVE - nn) / 3u;
             take = nn < 64u ? nn : 64u;
             take = take < room ? take : room;
         }
@@ -472,8 +478,7 @@ __device__ __forceinline__ void collect_pass(const SetDesc& sd, uint32_t set_ind
     if (sub == 0) {
         L.vroot[qs] = vr;
         L.vdepth[qs] = vdepth;
-        L.vcoll[qs] = vcoll;
-        L.vcell[qs] = vcell;
+        L.vcell[qs] = vcell | (vcoll ? VR_COLLECTS : 0u);
         L.vbox[qs] = vbox;
         L.mat[qs][0] = m.a; L.mat[qs][1] = m.b; L.mat[qs][2] = m.c;
         L.mat[qs][3] = m.d; L.mat[qs][4] = m.e; L.mat[qs][5] = m.f;
@@ -535,7 +540,7 @@ __device__ __forceinline__ void collect_pass(const SetDesc& sd, uint32_t set_ind
     }
     // every query's ff poles / piers from the root and edges from its virtual root.  A cursor
     // walks the entities 64 at a time; those that may add to their query's set take the next
-    // free places of the traversal (WL_ENT), which runs when they are full or the entities end
+    // free places of the traversal (CL_ENT), which runs when they are full or the entities end
     const uint32_t n_ff = it.n_poles + it.n_piers;
     const uint32_t n_ent = n_ff + it.n_v;
     const uint32_t total = P * n_ent;
@@ -544,7 +549,7 @@ __device__ __forceinline__ void collect_pass(const SetDesc& sd, uint32_t set_ind
     while (cursor < total) {
         wave_sync();
         uint32_t n = 0;
-        while (cursor < total && n < WL_ENT) {
+        while (cursor < total && n < CL_ENT) {
             const uint32_t e = cursor + lane;
             bool want = false;
             uint32_t root = 0, em = SURR_MASK, qi = 0;
@@ -592,16 +597,17 @@ __device__ __forceinline__ void collect_pass(const SetDesc& sd, uint32_t set_ind
                             }
                         }
                         want = (em & ~L.cdone[qi]) != 0u;
-                        if (want && !L.vcoll[qi])
+                        const uint32_t vc = L.vcell[qi];
+                        if (want && !(vc & VR_COLLECTS))
                             root = collect_start_cells(c, entity_box(false, gm), root, L.vdepth[qi], L.vbox[qi],
-                                                       L.vcell[qi] & 0xffffu, L.vcell[qi] >> 16);
+                                                       vc & 0xffffu, (vc >> 16) & 0x7fffu);
                     }
                 }
             }
-            // the first WL_ENT - n wanted entities of these 64 take places n, n + 1, ...
+            // the first CL_ENT - n wanted entities of these 64 take places n, n + 1, ...
             const uint64_t wb = __ballot(want);
             const uint32_t rank = lanes_below(wb, lane);
-            const uint32_t room = WL_ENT - n;
+            const uint32_t room = CL_ENT - n;
             const uint32_t n_want = (uint32_t)__builtin_popcountll(wb);
             if (want && rank < room) {
                 const uint32_t o = n + rank;
@@ -618,7 +624,7 @@ __device__ __forceinline__ void collect_pass(const SetDesc& sd, uint32_t set_ind
                 uint64_t r = wb;
                 for (uint32_t t = 0; t < room; t++) r &= r - 1ull;
                 cursor += (uint32_t)__builtin_ctzll(r);
-                n = WL_ENT;
+                n = CL_ENT;
             }
         }
         wave_collect(c, L, n, lane);
@@ -629,6 +635,11 @@ __device__ __forceinline__ void collect_pass(const SetDesc& sd, uint32_t set_ind
     // CG lanes (containment()'s result).  Only candidates can collide; with more than NC the
     // virtual root's entries are scanned.
     if (active && !L.ovf[qs]) {
+        // the query's transformation, bbox and virtual root entries from LDS, so that no
+        // registers hold them over the traversal
+        const Mat m{L.mat[qs][0], L.mat[qs][1], L.mat[qs][2], L.mat[qs][3], L.mat[qs][4], L.mat[qs][5]};
+        const float4 bb = L.qbox[qs];
+        const uint32_t vcnt = L.vcnt[qs], ve0 = L.vbeg[qs];
         const float2 qpoi = xform(m, it.poi.x, it.poi.y);
         const uint32_t nc = L.ncand[qs];
         const bool listed = nc <= NC;
@@ -694,7 +705,8 @@ __global__ void k_collect_reset(uint32_t* counters) {
     for (uint32_t i = threadIdx.x; i < N_COUNTER_WORDS; i += blockDim.x) counters[i] = 0u;
 }
 
-__global__ __launch_bounds__(WAVE) void k_collect_units(const SetDesc* __restrict__ sets, const Unit* __restrict__ units,
+// CL_WAVES waves per SIMD (the VGPR limit that sets), 4 CL_WAVES per CU.
+__global__ __launch_bounds__(WAVE, CL_WAVES) void k_collect_units(const SetDesc* __restrict__ sets, const Unit* __restrict__ units,
                                                        uint32_t n_units, uint32_t* __restrict__ counters,
                                                        const float* __restrict__ m6, uint32_t cap,
                                                        uint32_t* __restrict__ count, uint64_t* __restrict__ hash,
@@ -724,7 +736,7 @@ jg_status collect_launch(const TableLayout& L, uint8_t* base, const float* d_m6,
     const SetDesc* sets = (const SetDesc*)base;
     const Unit* units = (const Unit*)(base + L.o_units);
     hipLaunchKernelGGL(k_collect_reset, dim3(1), dim3(64), 0, stream, counters);
-    const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(n_cus * 16u, L.n_units));
+    const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(n_cus * 4u * CL_WAVES, L.n_units));
     hipLaunchKernelGGL(k_collect_units, dim3(grid), dim3(WAVE), 0, stream, sets, units, L.n_units, counters, d_m6,
                        d_keys ? cap : 0u, d_count, d_hash, d_keys, ovf);
     jg_status st = hip_check(hipGetLastError(), "batched collection launch");
