@@ -32,7 +32,7 @@ using namespace jg;
 
 namespace {
 
-// LDS of a workgroup (one wavefront): its work list
+// LDS of a workgroup (one wavefront): its work list (within 8 KB, so that 20 waves share a CU)
 struct WaveLds {
     uint32_t nodes[WL_NCAP];
     uint32_t edges[WL_ECAP];
@@ -42,6 +42,7 @@ struct WaveLds {
     uint8_t edge_ent[WL_ECAP];
     uint8_t ent_slot[WL_ENT];
 };
+static_assert(sizeof(WaveLds) <= 8192, "20 waves of a CU share its 160 KB of LDS");
 
 __device__ __forceinline__ WorkList wave_list(WaveLds& L) {
     return WorkList{L.nodes, L.edges, L.edge_ent, L.geom, L.ent_slot, L.hit, L.qidx};
@@ -119,7 +120,7 @@ __device__ __forceinline__ uint64_t surrogate_unit(const SetDesc& sd, const Unit
 }
 
 template <bool FILT, int KIND>
-__global__ __launch_bounds__(WAVE) void k_surrogate(const SetDesc* __restrict__ sets, const Unit* __restrict__ units,
+__global__ __launch_bounds__(WAVE, 5) void k_surrogate(const SetDesc* __restrict__ sets, const Unit* __restrict__ units,
                                                    uint32_t n_units, uint32_t* __restrict__ counters,
                                                    Group* __restrict__ groups, const uint32_t* __restrict__ gbase,
                                                    const float* __restrict__ m6, uint8_t* __restrict__ status) {
@@ -449,7 +450,7 @@ __device__ __forceinline__ void poly_loop(const SetDesc* __restrict__ sets, uint
 // that keep 4 waves per SIMD; their kernels (and the fused kernel, whose pier-entity instance
 // lands at 129) are bounded to that occupancy at the cost of a few spills, which measured 16%
 // faster on the filtered many-layout workload than 3 waves without spills.
-__global__ __launch_bounds__(WAVE) void k_poly(const SetDesc* __restrict__ sets, uint32_t* __restrict__ counters,
+__global__ __launch_bounds__(WAVE, 5) void k_poly(const SetDesc* __restrict__ sets, uint32_t* __restrict__ counters,
                                               const Group* __restrict__ groups, const uint32_t* __restrict__ gbase,
                                               const float* __restrict__ m6, uint8_t* __restrict__ status) {
     __shared__ WaveLds lds;
@@ -628,7 +629,7 @@ jg_status table_image(const std::vector<jg_query_set>& sets, const std::vector<I
     std::memcpy(host.data() + L.o_units, units.data(), sizeof(Unit) * units.size());
     std::memcpy(host.data() + L.o_gbase, gbase, sizeof(gbase));
     // one wavefront per workgroup, each taking one work unit (or survivor group) at a time
-    const uint32_t want = cfg.workgroups ? cfg.workgroups : n_cus * 16u;
+    const uint32_t want = cfg.workgroups ? cfg.workgroups : n_cus * 20u;
     L.grid_s = std::max<uint32_t>(1u, std::min<uint32_t>(want, L.n_units));
     L.grid_p = std::max<uint32_t>(1u, (uint32_t)std::min<uint64_t>(want, max_groups));
     return JG_OK;
@@ -1068,9 +1069,9 @@ extern "C" jg_status jg_plan_autotune(jg_plan* p, const float* d_m6, uint8_t* d_
     if (st == JG_OK) st = plan_time(p, d_m6, d_status, reps, s, a, b, cur_ms);
     const uint32_t cus = p->n_cus ? p->n_cus : 256u;
     const std::vector<uint32_t> lanes = {0u, 8u, 16u, 32u, 64u};
-    const std::vector<uint32_t> wgs = {4u * cus, 8u * cus, 16u * cus, 32u * cus};
+    const std::vector<uint32_t> wgs = {4u * cus, 8u * cus, 16u * cus, 20u * cus, 32u * cus};
     const std::vector<uint32_t> chunks = {16u, 32u, 64u};
-    // the whole grid (60 two-kernel and 60 fused schedules): timings are noisy enough that
+    // the whole grid (75 two-kernel and 75 fused schedules): timings are noisy enough that
     // coordinate descent can stop in a poor corner
     for (uint32_t sv : {2u, 1u})
     for (uint32_t lv : lanes)

@@ -344,7 +344,7 @@ enum : int { KIND_EDGE = 0, KIND_CIRCLE = 1, KIND_MIXED = 2 };
 
 constexpr uint32_t WL_ENT = 128;          // entities per traversal
 constexpr uint32_t WL_NCAP = 512;         // node-visit queue entries per wave
-constexpr uint32_t WL_ECAP = 1024;        // edge-test queue entries per wave
+constexpr uint32_t WL_ECAP = 640;         // edge-test queue entries per wave (LDS within 8 KB: 20 waves per CU)
 constexpr int WL_EDGE_BITS = 10;          // edge tests one leaf visit pushes: fewer than 2^WL_EDGE_BITS
 static_assert(64u * ((1u << WL_EDGE_BITS) - 1u) < 65536u, "a wave's edge pushes must fit 16 bits");
 constexpr uint32_t WL_RESERVE = 3u * (JG_MAX_QT_DEPTH + 1u);   // node slots a depth-first walk may need

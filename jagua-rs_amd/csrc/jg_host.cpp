@@ -423,7 +423,7 @@ int32_t QuadTree::alloc_children(int32_t parent) {
         if (!(c.bbox.x < c.bbox.z && c.bbox.y < c.bbox.w)) c.bbox = c.bbox;  // degenerate boxes are kept as is
     }
     pool[parent].child = base;
-    if ((uint32_t)(pool[0].level - pool[parent].level) < CELL_LEVELS_MAX) near_root_changed = true;
+    if ((uint32_t)(pool[0].level - pool[parent].level) < CELL_LEVELS_MAX) cells_dirty.push_back(parent);
     touched.push_back(parent);
     for (int k = 0; k < 4; k++) touched.push_back(base + k);
     return base;
@@ -436,7 +436,7 @@ void QuadTree::free_subtree_children(int32_t node) {
     for (int k = 0; k < 4; k++) pool[base + k] = QNode{};
     free_blocks.push_back(base);
     pool[node].child = -1;
-    if ((uint32_t)(pool[0].level - pool[node].level) < CELL_LEVELS_MAX) near_root_changed = true;
+    if ((uint32_t)(pool[0].level - pool[node].level) < CELL_LEVELS_MAX) cells_dirty.push_back(node);
     touched.push_back(node);
 }
 
@@ -778,7 +778,7 @@ void image_build(PoolImage& im, QuadTree& qt, const SlotMap& map) {
     for (uint32_t s = 1; s < S; s++)
         if (map.slots[s].occ) image_write_slot(im, map, s);
     image_cells(im, qt);
-    qt.near_root_changed = false;
+    qt.cells_dirty.clear();
     im.cap_nodes = with_headroom(N);
     im.cap_ent = with_headroom(im.ent.size());
     im.cap_edges = with_headroom(im.edges.size());
@@ -832,7 +832,7 @@ bool image_patch(PoolImage& im, QuadTree& qt, const SlotMap& map, std::vector<ui
         const FlatNode& old = im.nodes[i];
         const bool was = old.first_child < 0 || old.n_active_edges <= qt.cd_threshold;
         if ((uint32_t)(qt.pool[0].level - qt.pool[i].level) <= CELL_LEVELS_MAX && was != collects_at(qt, (int32_t)i))
-            qt.near_root_changed = true;
+            qt.cells_dirty.push_back((int32_t)i);
         image_write_node(im, qt, map, i);
     }
     const uint32_t S = (uint32_t)map.slots.size();
@@ -876,13 +876,41 @@ bool image_patch(PoolImage& im, QuadTree& qt, const SlotMap& map, std::vector<ui
     }
     if (im.verts.size() > verts0)
         add(im.o_vv + sizeof(float2) * (uint64_t)verts0, im.verts.data() + verts0, sizeof(float2) * (im.verts.size() - verts0));
-    if (qt.near_root_changed) {
-        // children made or dropped within the tables' levels change the cell tables
-        image_cells(im, qt);
-        add(im.o_ce, im.cells.data(), sizeof(uint32_t) * im.cells.size());
+    // the cells under each dirty node: its depth, cell and the first collecting depth above it
+    // from the parent links, then its block of each level from its own down, rewritten and
+    // uploaded row by row (the whole tables when the root is dirty)
+    for (int32_t x : qt.cells_dirty) {
+        uint32_t d = 0, ix = 0, iy = 0, top = 0;
+        bool above = false, live = true;
+        for (uint32_t n = (uint32_t)x; n != 0u && live; d++) {
+            const uint32_t up = im.up[n], k = up & 3u, p = up >> 2;
+            // a node dropped since it was marked is no longer its parent's child: skip it
+            live = d < CELL_LEVELS_MAX && qt.pool[p].child >= 0 && (uint32_t)qt.pool[p].child + k == n;
+            ix |= (k == 0 || k == 3 ? 1u : 0u) << d;
+            iy |= (k == 0 || k == 1 ? 1u : 0u) << d;
+            n = p;
+            if (collects_at(qt, (int32_t)n)) {
+                above = true;
+                top = d + 1u;                     // steps above x of the topmost collecting ancestor so far
+            }
+        }
+        if (!live || d > im.cell_level) continue;
+        fill_cells(im, qt, x, d, d, ix, iy, above ? d - top : CELL_NO_CDEPTH);
+        if (d == 0) {
+            add(im.o_ce, im.cells.data(), sizeof(uint32_t) * im.cells.size());
+            break;
+        }
+        for (uint32_t l = d; l <= im.cell_level; l++) {
+            const uint32_t s = l - d;
+            const uint64_t off = (((uint64_t)1 << (2 * l)) - 1) / 3;
+            for (uint32_t y = iy << s; y < (iy + 1u) << s; y++) {
+                const uint64_t at = off + ((uint64_t)y << l) + ((uint64_t)ix << s);
+                add(im.o_ce + sizeof(uint32_t) * at, &im.cells[at], sizeof(uint32_t) << s);
+            }
+        }
     }
     qt.touched.clear();
-    qt.near_root_changed = false;
+    qt.cells_dirty.clear();
     return true;
 }
 

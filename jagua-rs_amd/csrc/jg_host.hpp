@@ -103,7 +103,9 @@ struct QuadTree {
     std::vector<QNode> pool;
     std::vector<int32_t> free_blocks;
     std::vector<int32_t> touched;        // nodes whose record, entries or children changed since the last image patch
-    bool near_root_changed = false;      // children made or dropped within the cell tables' levels since then
+    // nodes within the cell tables' levels whose cells changed since the last image build or
+    // patch: children made or dropped, or the node started or stopped collecting
+    std::vector<int32_t> cells_dirty;
     uint8_t cd_threshold = 16;
     void init(uint8_t depth, float4 bbox, uint8_t cdt);
     void register_hazard(int32_t node, QEntry e, const SlotMap& map);
