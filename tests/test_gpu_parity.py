@@ -154,8 +154,8 @@ def test_collect_sets_beyond_the_slot_list(cuda):
 def test_random_update_sequences(cuda, depth):
     """Random batches of deregister / register / save-restore (lbf/tests/tests.rs:48-66 in a
     loop).  Each batch reaches the device image as an in-place patch of the touched nodes and
-    slots; after every batch the plain detect / collect and a batched plan agree with the
-    oracle bit for bit, and the keys handed out agree with the oracle's slot map."""
+    slots; after every batch the plain detect / collect and a batched plan's detect and collect
+    agree with the oracle bit for bit, and the keys handed out agree with the oracle's slot map."""
     sc = S.config1_scene(40)
     sc.depth = depth
     cde, keys, oc = engines(sc)
@@ -194,5 +194,10 @@ def test_random_update_sequences(cuda, depth):
         st, ref = run_both(cde, oc, item, oitem, mats, cuda=cuda)
         plan.detect(m, out)
         np.testing.assert_array_equal(out.cpu().numpy(), ref["status"])
+        # the batched collection starts query edges through the cell tables' first collecting
+        # depths, which the patches keep current
+        cnt, hsh, _ = plan.collect(m, capacity=0)
+        np.testing.assert_array_equal(cnt.cpu().numpy().astype(np.uint32), ref["count"])
+        np.testing.assert_array_equal(hsh.cpu().numpy().view(np.uint64), ref["hash"])
     s = cde.stats()
     assert s["n_patches"] >= 8 and s["n_full_uploads"] - full0 <= 2, s
