@@ -132,9 +132,10 @@ struct PairGrad {
 // each chunk padded with NaN poles (which overlap nothing) to a multiple of 8.  The lanes form
 // an 8 x 8 grid: lane 8u + w takes the pole pairs (u + 8s, w + 8t) of the chunk pair, so the
 // loops are the same for every lane; a lane holds its (at most 8) poles of b in registers and
-// reads one pole of a from LDS per 8 pole pairs.  Pole pairs that overlap are rare, so their terms sit behind a wave-uniform test (a
-// ballot) that skips them for the whole wave when no lane has one.  The two items' sin/cos
-// come from one call: lane 0 takes theta_a, the other lanes theta_b.
+// reads one pole of a from LDS per 8 pole pairs.  Pole pairs that overlap are rare, so their
+// terms sit behind a wave-uniform test (a ballot) that skips them for the whole wave when no
+// lane has one.  The two items' sin/cos come from one call: lane 0 takes theta_a, the other
+// lanes theta_b.
 template <int MODE>
 __device__ __forceinline__ void pair_term(const OvView& v, const float* __restrict__ pose3, uint64_t pid,
                                           uint32_t lane, float4* la, float4* lb, float& acc, PairGrad& g) {
