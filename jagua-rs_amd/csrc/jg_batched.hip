@@ -446,17 +446,18 @@ __device__ __forceinline__ void poly_loop(const SetDesc* __restrict__ sets, uint
     if (last_wave_out(counters, lane) && lane < N_STREAMS) counters[C_GCOUNT0 + STREAM_STRIDE * lane] = 0u;
 }
 
-// The filtered instances carry the filter state on top and would need a few VGPRs over the 128
-// that keep 4 waves per SIMD; their kernels (and the fused kernel, whose pier-entity instance
-// lands at 129) are bounded to that occupancy at the cost of a few spills, which measured 16%
-// faster on the filtered many-layout workload than 3 waves without spills.
+// The two-stage kernels run 5 waves per SIMD (96 VGPRs, a few spilled; the LDS of a wave is
+// within 8 KB).  The filtered polygon stage carries the filter state on top and spills more at
+// 96, and still measured 4% faster on the filtered many-layout workload than at 4 waves; the
+// fused kernels stay bounded to 4 (128 VGPRs: their pier-entity instance lands at 129), which
+// measured 16% faster there than 3 waves without spills.
 __global__ __launch_bounds__(WAVE, 5) void k_poly(const SetDesc* __restrict__ sets, uint32_t* __restrict__ counters,
                                               const Group* __restrict__ groups, const uint32_t* __restrict__ gbase,
                                               const float* __restrict__ m6, uint8_t* __restrict__ status) {
     __shared__ WaveLds lds;
     poly_loop<false>(sets, counters, groups, gbase, m6, status, lds);
 }
-__global__ __launch_bounds__(WAVE, 4) void k_poly_filtered(const SetDesc* __restrict__ sets, uint32_t* __restrict__ counters,
+__global__ __launch_bounds__(WAVE, 5) void k_poly_filtered(const SetDesc* __restrict__ sets, uint32_t* __restrict__ counters,
                                                           const Group* __restrict__ groups,
                                                           const uint32_t* __restrict__ gbase, const float* __restrict__ m6,
                                                           uint8_t* __restrict__ status) {
