@@ -400,7 +400,7 @@ jg_status jg_overlap_create(const float* poles, const uint32_t* pole_off, uint32
                             uint64_t n_pairs, const float* cedges, uint32_t n_cedges, jg_overlap** out);
 /* Replaces the candidate pairs (host [n_pairs][2]). */
 jg_status jg_overlap_set_pairs(jg_overlap* ov, const uint32_t* pairs, uint64_t n_pairs);
-/* Forward: d_pose device [n_items][3] (x, y, theta); d_q device f32[1]. */
+/* Forward: d_pose device [n_items][3] (x, y, theta; may be NULL when n_items is 0); d_q device f32[1]. */
 jg_status jg_overlap_fwd(jg_overlap* ov, const float* d_pose, float* d_q, void* stream);
 /* Backward: dQ/d(x, y, theta) per item into d_grad device [n_items][3]. */
 jg_status jg_overlap_bwd(jg_overlap* ov, const float* d_pose, float* d_grad, void* stream);

@@ -761,17 +761,17 @@ jg_status ov_launch(jg_overlap* ov, const float* d_pose, float* d_q, float* d_gr
 }  // namespace
 
 extern "C" jg_status jg_overlap_fwd(jg_overlap* ov, const float* d_pose, float* d_q, void* stream) {
-    if (!ov || !d_pose || !d_q) return fail(JG_ERR_INVALID_ARG, "null argument");
+    if (!ov || !d_q || (ov->n_items && !d_pose)) return fail(JG_ERR_INVALID_ARG, "null argument");
     return ov_launch<OV_Q>(ov, d_pose, d_q, nullptr, (hipStream_t)stream);
 }
 
 extern "C" jg_status jg_overlap_bwd(jg_overlap* ov, const float* d_pose, float* d_grad, void* stream) {
-    if (!ov || !d_pose || !d_grad) return fail(JG_ERR_INVALID_ARG, "null argument");
+    if (!ov || (ov->n_items && (!d_pose || !d_grad))) return fail(JG_ERR_INVALID_ARG, "null argument");
     return ov_launch<OV_GRAD>(ov, d_pose, nullptr, d_grad, (hipStream_t)stream);
 }
 
 extern "C" jg_status jg_overlap_fwd_bwd(jg_overlap* ov, const float* d_pose, float* d_q, float* d_grad, void* stream) {
-    if (!ov || !d_pose || !d_q || !d_grad) return fail(JG_ERR_INVALID_ARG, "null argument");
+    if (!ov || !d_q || (ov->n_items && (!d_pose || !d_grad))) return fail(JG_ERR_INVALID_ARG, "null argument");
     return ov_launch<OV_Q | OV_GRAD>(ov, d_pose, d_q, d_grad, (hipStream_t)stream);
 }
 

@@ -67,6 +67,39 @@ def test_overlap_items_of_many_poles(cuda):
     assert np.array_equal(ov.backward(pose).cpu().numpy().astype(np.float64), g)
 
 
+def test_overlap_degenerate_shapes(cuda):
+    """No pairs, no container edges, items without poles and items of one pole: every form
+    returns what the FP64 oracle does (Q = 0 and a zero gradient where nothing overlaps)."""
+    rng = np.random.default_rng(11)
+    n_items = 12
+    counts = np.array([0, 1, 1, 3, 0, 5, 1, 2, 0, 7, 1, 4], dtype=np.uint32)
+    off = np.zeros(n_items + 1, dtype=np.uint32)
+    off[1:] = np.cumsum(counts)
+    poles = np.stack([rng.uniform(-2, 2, off[-1]), rng.uniform(-2, 2, off[-1]), rng.uniform(0.5, 2.0, off[-1])],
+                     axis=1).astype(np.float32)
+    pose = np.stack([rng.uniform(0, 12, n_items), rng.uniform(0, 6, n_items), rng.uniform(0, 6.28, n_items)],
+                    axis=1).astype(np.float32)
+    pairs = np.array([(i, j) for i in range(n_items) for j in range(i + 1, n_items)], dtype=np.uint32)
+    box = np.array([[0, 0, 12, 0], [12, 0, 12, 6], [12, 6, 0, 6], [0, 6, 0, 0]], dtype=np.float32)
+    pt = torch.from_numpy(pose).to(cuda)
+    for prs, ce in [(pairs[:0], box[:0]), (pairs[:0], box), (pairs, box[:0]), (pairs, box)]:
+        ov = J.PoleOverlap(poles, off, prs, ce)
+        q, g = ov.forward_backward(pt)
+        q, g = q.item(), g.cpu().numpy().astype(np.float64)
+        q64, g64 = O.overlap64(pose.astype(np.float64), poles, off, prs, ce)
+        if q64 == 0.0:
+            assert q == 0.0 and not g.any(), (len(prs), len(ce), q, np.abs(g).max())
+        else:
+            assert abs(q - q64) / q64 < 1e-5, (len(prs), len(ce), q, q64)
+            assert np.linalg.norm(g - g64) / np.linalg.norm(g64) < 1e-4
+        assert ov.forward(pt).item() == q
+        assert np.array_equal(ov.backward(pt).cpu().numpy().astype(np.float64), g)
+    empty = J.PoleOverlap(poles[:0], np.zeros(1, dtype=np.uint32), pairs[:0], box)
+    assert empty.forward(pt[:0]).item() == 0.0                 # an empty pose tensor has no address
+    q0, g0 = empty.forward_backward(pt[:0])
+    assert q0.item() == 0.0 and g0.numel() == 0
+
+
 @pytest.mark.parametrize("h,tol", [(1e-3, 2e-3), (1e-5, 1e-4)])
 def test_overlap_backward_vs_central_differences(cuda, h, tol):
     w = S.config4_workload(seed=6, n_items=64, width=250.0, height=100.0)
