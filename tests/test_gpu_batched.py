@@ -190,3 +190,32 @@ def test_batched_shallow_virtual_roots(cuda, filtered):
         out = torch.full((first,), 9, dtype=torch.uint8, device=cuda)
         J.QueryPlan(sets, stages=stages).detect(m, out)
         np.testing.assert_array_equal(out.cpu().numpy(), ref)
+
+
+def test_empty_and_ragged_sets(cuda):
+    """Query sets of zero queries among non-empty ones (ragged), a plan of only empty sets and
+    empty plain batches: the other sets' verdicts and hazard sets are the oracle's, and the empty
+    ones launch nothing that writes."""
+    sc = S.config1_scene(23)
+    cde, keys, oc = engines(sc)
+    polys, dts = S.config1_queries(23, n=600)
+    prod, orc = items(polys)
+    mats = J.compose(dts)
+    m = torch.from_numpy(mats).to(cuda)
+    ref = O.query(oc, orc[0], mats=mats, mode=3)
+    sets = [(cde, prod[0], 0, 0, None), (cde, prod[0], 0, 250, None), (cde, prod[0], 250, 0, None),
+            (cde, prod[0], 250, 350, None), (cde, prod[0], 600, 0, None)]
+    plan = J.QueryPlan(sets)
+    out = torch.full((len(mats),), 255, dtype=torch.uint8, device=cuda)
+    plan.detect(m, out)
+    np.testing.assert_array_equal(out.cpu().numpy(), ref["status"])
+    cnt, hsh, _ = plan.collect(m, capacity=0)
+    np.testing.assert_array_equal(cnt.cpu().numpy().astype(np.uint32), ref["count"])
+    np.testing.assert_array_equal(hsh.cpu().numpy().view(np.uint64), ref["hash"])
+    # only empty sets, and empty plain batches
+    none = J.QueryPlan([(cde, prod[0], 0, 0, None)])
+    none.detect(m[:0], out[:0])
+    assert cde.detect_batch(prod[0], m[:0]).numel() == 0
+    c0, h0, _ = cde.collect_batch(prod[0], m[:0], capacity=0)
+    assert c0.numel() == 0 and h0.numel() == 0
+    torch.cuda.synchronize()
