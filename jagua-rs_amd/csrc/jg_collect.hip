@@ -4,8 +4,9 @@
 // one-wave workgroups that claim the plan's work units.
 //
 // A wave collects CP queries of a unit at a time.  All their entities (ff poles and piers from
-// the root, query edges from each query's virtual root) share one wave traversal in the manner
-// of the detection's (node visits and edge tests queued in LDS, 64 per step), with
+// the root, query edges from the first collecting node of their root path below the query's
+// virtual root, collect_start_cells) share one wave traversal in the manner of the detection's
+// (node visits and edge tests queued in LDS, 64 per step), with
 // QTNode::collect_collisions (qt_node.rs:151-191) as the visit: a node whose active edges number
 // at most cd_threshold, or a leaf, inserts its Entire hazards and queues edge tests for its
 // Partial hazards; other nodes queue their colliding quadrants.  Each query's collector is a
@@ -28,7 +29,7 @@
 //    is skipped when no entry outside the set may be collected by the entity's box.
 // The candidates also hold every hazard the containment loop can add (it needs a bbox
 // relation, or the exterior), so that loop runs over them.  On the configuration-2 layout
-// the skip takes the oracle's 525 node visits per query to about 150.
+// the skip takes the oracle's 525 node visits per query to about 196, and the start nodes to 78.
 //
 // A query whose set outgrows the LDS list (SET_CAP slots) is handed to the per-query collector
 // (k_collect_overflow), which keeps a bit per slot.  A query with more than NC candidates skips
