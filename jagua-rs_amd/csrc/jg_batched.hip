@@ -337,9 +337,17 @@ __device__ __forceinline__ void poly_queries(const SetDesc& sd, uint32_t q0, uin
         } else if (active) {
             vr = virtual_root_cells(c, root_bb, bb, vr_depth);
         }
+        // the virtual root's box and cell, below which the edges' start nodes are followed
+        float4 vbox = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        uint32_t vx = 0, vy = 0;
+        const bool vstart = active && vr_depth < c.cell_level;
+        if (vstart) {
+            vbox = ldg_node(c.nodes + vr).bbox;
+            node_cell(root_bb, vbox, vr_depth, vx, vy);
+        }
         // edges of the pass's unfinished queries, descending from their virtual roots
         // (cd_engine.rs:170-175), each from the deepest node below its virtual root known to
-        // hold it (start_node); a query slot is marked hit as soon as one edge collides
+        // hold it (start_node_below); a query slot is marked hit as soon as one edge collides
         wave_sync();
         if (sub == 0) {
             W.hit[slot] = done ? 1u : 0u;
@@ -364,10 +372,7 @@ __device__ __forceinline__ void poly_queries(const SetDesc& sd, uint32_t q0, uin
                     W.geom[o] = make_float4(a.x, a.y, b2.x, b2.y);
                     W.ent_slot[o] = (uint8_t)slot;
                     const float4 eb = make_float4(mn(a.x, b2.x), mn(a.y, b2.y), mx(a.x, b2.x), mx(a.y, b2.y));
-                    // both hold the edge, so one is on the other's root path: the deeper one
-                    uint32_t depth;
-                    const uint32_t sn = start_node(c, root_bb, eb, 0.0f, depth);
-                    if (depth > vr_depth) start = sn;
+                    if (vstart) start = start_node_below(c, (uint32_t)vr, vr_depth, vbox, vx, vy, eb);
                 }
                 n = wave_push_roots(W, n, want, o, start, lane);
             }

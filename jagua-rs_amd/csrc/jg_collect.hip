@@ -463,17 +463,8 @@ __device__ __forceinline__ void collect_pass(const SetDesc& sd, uint32_t set_ind
         vcnt = vn.haz_info & COUNT_MASK;
         vbox = vn.bbox;
         vcoll = (vn.first_child < 0 || vn.n_active_edges <= c.cd_threshold) ? 1u : 0u;
-        // vr's cell: the quadrants of the root box holding vr's centre (vr's box is one of them)
-        const float cx = (vbox.x + vbox.z) / 2.0f, cy = (vbox.y + vbox.w) / 2.0f;
-        float4 r = root_bb;
         uint32_t ix = 0, iy = 0;
-        for (uint32_t l = 0; l < vdepth && l < c.cell_level; l++) {
-            const float mxm = (r.x + r.z) / 2.0f, mym = (r.y + r.w) / 2.0f;
-            const bool xh = cx > mxm, yh = cy > mym;
-            r = make_float4(xh ? mxm : r.x, yh ? mym : r.y, xh ? r.z : mxm, yh ? r.w : mym);
-            ix = 2u * ix + (xh ? 1u : 0u);
-            iy = 2u * iy + (yh ? 1u : 0u);
-        }
+        if (vdepth < c.cell_level) node_cell(root_bb, vbox, vdepth, ix, iy);   // (else unused)
         vcell = (iy << 16) | ix;
     }
     if (sub == 0) {

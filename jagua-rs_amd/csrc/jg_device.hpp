@@ -236,39 +236,52 @@ __device__ __forceinline__ int virtual_root_cells(const CdeView& c, float4 root,
     }
 }
 
-// A node from which the descent of an entity with bounding box eb may start instead of the
-// root, and its depth: following the root box's quadrant subdivision (box_quadrant's
-// midpoints, the boxes the tree's nodes have) while eb lies strictly inside one quadrant, by at
-// least m, down to the cell tables' deepest level, the cell table gives the deepest node on
-// that path.  No memory is read until that one entry.
-//
-// QTNode::collides from the root visits this node with the same outcome.  Every hazard present
-// below a node is an entry of it (qt_node.rs:39-81: hazards reach children by constrict, and
-// an Entire hazard is Entire in every quadrant, qt_hazard.rs:58), and Entire entries come first
-// (qt_hazard_vec.rs), so along the root path: an ancestor without relevant hazards leaves this
-// node none either, and an ancestor whose strongest relevant hazard is Entire leaves it Entire
-// here too.  An entity strictly inside each quadrant of the path collides with no quadrant off
-// the path at any ancestor: for an edge, the box test of collides_with_quadrants finds those
-// quadrants disjoint (qt_traits.rs:19-106); for a circle, the margin m exceeds the rounding of
-// circle_box's distance (circle.rs:115-127).  So the ancestors' visits reduce to this one.
-__device__ __forceinline__ uint32_t start_node(const CdeView& c, float4 root, float4 eb, float m, uint32_t& depth) {
-    depth = 0;
-    if (c.cell_level == 0u) return 0u;
-    if (!(eb.x - m > root.x && eb.z + m < root.z && eb.y - m > root.y && eb.w + m < root.w)) return 0u;
+// The cell (ix, iy) at level `depth` of a node whose box `box` is a quadrant `depth` levels below
+// the root box: the quadrants holding the box's centre (strictly on one side of every midline
+// above it, as the box is one of the subdivision's).
+__device__ __forceinline__ void node_cell(float4 root, float4 box, uint32_t depth, uint32_t& ix, uint32_t& iy) {
+    const float cx = (box.x + box.z) / 2.0f, cy = (box.y + box.w) / 2.0f;
     float4 r = root;
-    uint32_t ix = 0, iy = 0, l = 0;
+    ix = 0;
+    iy = 0;
+    for (uint32_t l = 0; l < depth; l++) {
+        const float mxm = (r.x + r.z) / 2.0f, mym = (r.y + r.w) / 2.0f;
+        const bool xh = cx > mxm, yh = cy > mym;
+        r = make_float4(xh ? mxm : r.x, yh ? mym : r.y, xh ? r.z : mxm, yh ? r.w : mym);
+        ix = 2u * ix + (xh ? 1u : 0u);
+        iy = 2u * iy + (yh ? 1u : 0u);
+    }
+}
+
+// A node from which the descent of an edge with bounding box eb, starting at a node n0 that
+// holds it (depth d0 below the cell tables' deepest level, box box0, cell (x0, y0)), may start
+// instead: following the quadrant subdivision below n0 (box_quadrant's midpoints, the boxes the
+// tree's nodes have) while eb lies strictly inside one quadrant, down to the cell tables'
+// deepest level, the cell table gives the deepest node on that path (n0 when eb meets one of
+// n0's midlines).  No memory is read until that one entry.
+//
+// QTNode::collides from n0 visits this node with the same outcome.  Every hazard present below
+// a node is an entry of it (qt_node.rs:39-81: hazards reach children by constrict, and an
+// Entire hazard is Entire in every quadrant, qt_hazard.rs:58), and Entire entries come first
+// (qt_hazard_vec.rs), so along the path: an ancestor without relevant hazards leaves this node
+// none either, and an ancestor whose strongest relevant hazard is Entire leaves it Entire here
+// too.  An edge strictly inside each quadrant of the path collides with no quadrant off the
+// path at any ancestor (the box test of collides_with_quadrants finds those quadrants
+// disjoint, qt_traits.rs:19-106), so the ancestors' visits reduce to this one.
+__device__ __forceinline__ uint32_t start_node_below(const CdeView& c, uint32_t n0, uint32_t d0, float4 box0,
+                                                     uint32_t x0, uint32_t y0, float4 eb) {
+    float4 r = box0;
+    uint32_t ix = x0, iy = y0, l = d0;
     for (; l < c.cell_level; l++) {
         const float mxm = (r.x + r.z) / 2.0f, mym = (r.y + r.w) / 2.0f;
-        const bool xh = eb.x - m > mxm, xl = eb.z + m < mxm, yh = eb.y - m > mym, yl = eb.w + m < mym;
+        const bool xh = eb.x > mxm, xl = eb.z < mxm, yh = eb.y > mym, yl = eb.w < mym;
         if (!(xh || xl) || !(yh || yl)) break;
         r = make_float4(xh ? mxm : r.x, yh ? mym : r.y, xh ? r.z : mxm, yh ? r.w : mym);
         ix = 2u * ix + (xh ? 1u : 0u);
         iy = 2u * iy + (yh ? 1u : 0u);
     }
-    if (l == 0u) return 0u;
-    const uint32_t e = ldg1(c.cells + ((1u << (2u * l)) - 1u) / 3u + ((iy << l) | ix));
-    depth = cell_depth(e);
-    return e & CELL_NODE;
+    if (l == d0) return n0;
+    return ldg1(c.cells + ((1u << (2u * l)) - 1u) / 3u + ((iy << l) | ix)) & CELL_NODE;
 }
 
 // QTNode::collides for a circle (qt_node.rs:99-147; Circle quadrant test qt_traits.rs:10-13)
