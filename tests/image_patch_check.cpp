@@ -117,6 +117,15 @@ static void verify(const PoolImage& im, const std::vector<uint8_t>& dev, const Q
         }
     }
     CHECK(i == f.nodes.size(), "reachable nodes %zu vs %zu", i, f.nodes.size());
+    // collecting is inherited: a child's active edges are a subset of its parent's, so every
+    // child of a node where collect_collisions tests entries tests them too (the collection's
+    // start nodes and the cell tables' first collecting depth rely on it)
+    for (size_t n = 0; n < qt.pool.size(); n++) {
+        const QNode& q = qt.pool[n];
+        if (q.child < 0 || !collects_at(qt, (int32_t)n)) continue;
+        for (int k = 0; k < 4; k++)
+            CHECK(collects_at(qt, q.child + k), "node %zu collects, its child %d does not", n, k);
+    }
     // the cell tables: entry (ix, iy) of level l is cdepth << 28 | depth << 24 | node for the
     // deepest node on the tree path to that cell, cdepth the depth of the path's first node where
     // collect_collisions tests entries (a leaf or at most cd_threshold active edges), 15 if none
