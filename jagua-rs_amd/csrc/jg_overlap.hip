@@ -305,7 +305,8 @@ __global__ __launch_bounds__(64 * OV_WAVES) void k_ov_pass(OvView v, const float
 // The results of a pass.  With OV_GRAD, blocks 0 .. ceil(n_items / 1024)-1 gather each item's
 // gradient in a fixed order: its container term, then its pair terms in adjacency order, which
 // are contiguous in pg (f64 accumulation).  With OV_Q, the last block sums the n partials into
-// Q in a fixed order in f64 (thread t takes t, t + 1024, ..., then a fixed tree).
+// Q in a fixed order in f64 (thread t takes t, t + 1024, ..., then a fixed butterfly per wave
+// and the 16 wave sums in order).
 template <int MODE>
 __global__ __launch_bounds__(1024) void k_ov_finish(uint32_t n_items, const uint32_t* __restrict__ adj_off,
                                                     const float4* __restrict__ pg, const float* __restrict__ cg3,
@@ -324,20 +325,22 @@ __global__ __launch_bounds__(1024) void k_ov_finish(uint32_t n_items, const uint
         grad[3 * item + 2] = (float)gt;
         return;
     }
-    __shared__ double sh[1024];
+    __shared__ double sh[16];
     double acc = 0.0;
     for (uint64_t i = threadIdx.x; i < n; i += 4096) {   // four loads in flight, added in index order
         float e[4];
         for (int u = 0; u < 4; u++) e[u] = i + 1024u * u < n ? partial[i + 1024u * u] : 0.0f;
         for (int u = 0; u < 4; u++) acc += (double)e[u];
     }
-    sh[threadIdx.x] = acc;
+    // a fixed butterfly within each wave, then the 16 wave sums in order by the first wave
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63u) == 0) sh[threadIdx.x >> 6] = acc;
     __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < 16; w++) t += sh[w];
+        q[0] = (float)t;
     }
-    if (threadIdx.x == 0) q[0] = (float)sh[0];
 }
 
 // ---- candidate pairs from the engine's quadtree --------------------------------------
